@@ -1,0 +1,86 @@
+/*
+ * szp_amd.h -- device-resident API of libszp_amd.so (SZp fastmode1 float32 on
+ * MI355X).  Plain pointers and sizes; `stream` is a hipStream_t passed as
+ * void* (NULL = the default stream).
+ *
+ * The *_dev entry points enqueue kernels and return at once: no host
+ * synchronisation and no allocation.  The caller provides device buffers:
+ * input/output arrays, the output stream (16-byte aligned) and a workspace
+ * (256-byte aligned) of the size the *_workspace_size queries return.  Calls
+ * that share a workspace must be ordered (same stream).  Errors found on the
+ * device (capacity, malformed stream) accumulate in the workspace and are
+ * read by szp_amd_check_status().
+ *
+ * Streams are byte-identical to the reference's szp_compress(SZP_NONRANDOMACCESS,
+ * SZ_FLOAT, ...) run with an OpenMP team of nbThreads threads
+ * (SZp/src/szp_float.cc:148-309); decompressed values are bit-identical to
+ * SZp/src/szpd_float.cc:23-128.
+ */
+#ifndef SZP_AMD_API_H
+#define SZP_AMD_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum szp_amd_status {
+    SZP_AMD_OK = 0,
+    SZP_AMD_ERR_ARG = 1,         /* invalid argument (sizes, modes, alignment) */
+    SZP_AMD_ERR_CAPACITY = 2,    /* output or workspace too small */
+    SZP_AMD_ERR_HIP = 3,         /* HIP runtime error */
+    SZP_AMD_ERR_STREAM = 4,      /* malformed compressed stream */
+    SZP_AMD_ERR_TIMEOUT = 5,     /* a device-side wait exceeded its bound */
+    SZP_AMD_ERR_UNSUPPORTED = 6, /* mode outside fastmode1 float32 */
+    SZP_AMD_ERR_NOMEM = 7        /* host or device allocation failed */
+} szp_amd_status;
+
+/* No reference counterpart (the reference mallocs 4N+4 and can overflow,
+ * SZp/src/szp_float.cc:156): the largest stream for (nbEle, blockSize, T). */
+uint64_t szp_amd_compress_bound(uint64_t nbEle, int blockSize, int nbThreads);
+
+uint64_t szp_amd_compress_workspace_size(uint64_t nbEle, int blockSize, int nbThreads);
+uint64_t szp_amd_decompress_workspace_size(uint64_t streamBytes, uint64_t nbEle, int blockSize,
+                                           int nbThreads);
+
+/* Device form of szp_compress(SZP_NONRANDOMACCESS, SZ_FLOAT, ...)
+ * (SZp/src/szp.cc:23-58): errBoundMode ABS (0) or REL (1).  Writes the
+ * stream to d_out and its length (u64) to *d_outSize. */
+int szp_amd_compress_dev(const float *d_data, uint64_t nbEle, int errBoundMode,
+                         double absErrBound, double relBoundRatio, int blockSize, int nbThreads,
+                         unsigned char *d_out, uint64_t outCapacity, uint64_t *d_outSize,
+                         void *d_work, uint64_t workBytes, void *stream);
+
+/* Device form of szp_decompress(SZP_NONRANDOMACCESS, SZ_FLOAT, ...)
+ * (SZp/src/szp.cc:118-139).  streamBytes is the stream length or any upper
+ * bound within the d_stream allocation. */
+int szp_amd_decompress_dev(const unsigned char *d_stream, uint64_t streamBytes, uint64_t nbEle,
+                           int blockSize, int nbThreads, float *d_out, void *d_work,
+                           uint64_t workBytes, void *stream);
+
+/* Synchronises `stream` and returns the status accumulated in d_work. */
+int szp_amd_check_status(void *d_work, void *stream);
+
+/* Exact length of a host-resident stream, from its header and block widths. */
+uint64_t szp_amd_stream_length(const unsigned char *bytes, uint64_t nbEle, int blockSize,
+                               int nbThreads);
+
+/* T used by the drop-in entry points of szp.h; set_threads(0) restores auto. */
+int szp_amd_default_threads(void);
+void szp_amd_set_threads(int nbThreads);
+
+/* Width of a block maximum as the reference computes it: 0 for 0, else
+ * (int)log2f((float)max) + 1 (SZp/src/szp_float.cc:253-262). */
+int szp_amd_bit_width(uint32_t maxMagnitude);
+
+const char *szp_amd_status_string(int status);
+/* Message of the last failure of a szp.h entry point in this thread. */
+const char *szp_amd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SZP_AMD_API_H */

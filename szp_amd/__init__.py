@@ -1,0 +1,229 @@
+"""szp_amd -- SZp fastmode1 float32 compress/decompress on MI355X.
+
+Python mirror of the reference's C entry points (SZp/include/szp.h:33-36),
+driven through the C ABI of the in-tree library ``libszp_amd.so`` (HIP kernels
+for gfx950), plus a device-resident form on torch tensors.
+
+    stream, size = szp_compress(SZP_NONRANDOMACCESS, SZ_FLOAT, data, ABS, 1e-3, 0.0,
+                                data.size, 128)
+    values = szp_decompress(SZP_NONRANDOMACCESS, SZ_FLOAT, stream, size, data.size, 128)
+
+There is no CPU fallback: importing this package fails when the library is
+missing, and every call runs the HIP kernels.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "ABS", "REL", "SZP_NONRANDOMACCESS", "SZP_RANDOMACCESS", "SZ_FLOAT", "SZ_DOUBLE",
+    "SzpError", "lib", "library_path", "szp_compress", "szp_decompress",
+    "compress_bound", "default_threads", "set_threads", "bit_width", "stream_length",
+    "DeviceCodec",
+]
+
+# SZp/include/szp_defines.h:19-20,33-37
+ABS = 0
+REL = 1
+SZP_NONRANDOMACCESS = 1
+SZP_RANDOMACCESS = 2
+SZ_FLOAT = 0
+SZ_DOUBLE = 1
+
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "capacity too small", 3: "HIP runtime error",
+    4: "malformed compressed stream", 5: "device wait timed out", 6: "unsupported mode",
+    7: "allocation failed",
+}
+
+library_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libszp_amd.so")
+
+
+class SzpError(RuntimeError):
+    def __init__(self, what, code=None):
+        msg = what if code is None else "%s: %s (%d)" % (what, STATUS.get(code, "?"), code)
+        super().__init__(msg)
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(library_path):
+        raise ImportError(
+            "libszp_amd.so is not built (%s); run `python -m szp_amd.build` or "
+            "__graft_entry__.build()" % library_path)
+    L = ctypes.CDLL(library_path)
+    u64, i32, vp, dbl, f32 = (ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
+                              ctypes.c_float)
+    szt = ctypes.c_size_t
+    sig = {
+        "szp_compress": (vp, [i32, i32, vp, ctypes.POINTER(szt), i32, dbl, dbl, szt, i32]),
+        "szp_decompress": (vp, [i32, i32, vp, szt, szt, i32]),
+        "szp_float_openmp_threadblock": (vp, [vp, ctypes.POINTER(szt), f32, szt, i32]),
+        "szp_float_openmp_threadblock_arg": (None, [vp, vp, ctypes.POINTER(szt), f32, szt, i32]),
+        "szp_float_decompress_openmp_threadblock": (vp, [szt, f32, i32, vp]),
+        "szp_float_decompress_openmp_threadblock_arg": (None, [vp, szt, f32, i32, vp]),
+        "szp_amd_compress_bound": (u64, [u64, i32, i32]),
+        "szp_amd_compress_workspace_size": (u64, [u64, i32, i32]),
+        "szp_amd_decompress_workspace_size": (u64, [u64, u64, i32, i32]),
+        "szp_amd_compress_dev": (i32, [vp, u64, i32, dbl, dbl, i32, i32, vp, u64, vp, vp, u64, vp]),
+        "szp_amd_decompress_dev": (i32, [vp, u64, u64, i32, i32, vp, vp, u64, vp]),
+        "szp_amd_check_status": (i32, [vp, vp]),
+        "szp_amd_stream_length": (u64, [vp, u64, i32, i32]),
+        "szp_amd_default_threads": (i32, []),
+        "szp_amd_set_threads": (None, [i32]),
+        "szp_amd_bit_width": (i32, [ctypes.c_uint32]),
+        "szp_amd_status_string": (ctypes.c_char_p, [i32]),
+        "szp_amd_last_error": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+_libc = ctypes.CDLL("libc.so.6")
+_libc.free.argtypes = [ctypes.c_void_p]
+_libc.free.restype = None
+
+
+def _last_error():
+    return (lib.szp_amd_last_error() or b"").decode()
+
+
+def default_threads():
+    """T used by szp_compress/szp_decompress (the reference's OpenMP team size)."""
+    return lib.szp_amd_default_threads()
+
+
+def set_threads(n):
+    """Pins T for the drop-in entry points; 0 restores the automatic choice."""
+    lib.szp_amd_set_threads(int(n))
+
+
+def compress_bound(n, block_size, threads):
+    return int(lib.szp_amd_compress_bound(int(n), int(block_size), int(threads)))
+
+
+def bit_width(m):
+    return int(lib.szp_amd_bit_width(int(m)))
+
+
+def stream_length(stream, n, block_size, threads):
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    return int(lib.szp_amd_stream_length(s.ctypes.data, int(n), int(block_size), int(threads)))
+
+
+def szp_compress(fastMode, dataType, data, errBoundMode, absErrBound, relBoundRatio, nbEle,
+                 blockSize):
+    """Mirror of szp_compress (SZp/include/szp.h:33-34): returns (stream, outSize).
+
+    The stream is a numpy uint8 array copied out of the library's malloc'd
+    buffer (which is then freed, as the reference's callers do)."""
+    x = np.ascontiguousarray(data, dtype=np.float32)
+    if x.size < nbEle:
+        raise ValueError("data holds %d elements, nbEle=%d" % (x.size, nbEle))
+    size = ctypes.c_size_t(0)
+    p = lib.szp_compress(int(fastMode), int(dataType), x.ctypes.data, ctypes.byref(size),
+                         int(errBoundMode), float(absErrBound), float(relBoundRatio), int(nbEle),
+                         int(blockSize))
+    if not p:
+        raise SzpError("szp_compress failed: " + _last_error())
+    try:
+        out = np.ctypeslib.as_array((ctypes.c_uint8 * size.value).from_address(p)).copy()
+    finally:
+        _libc.free(p)
+    return out, size.value
+
+
+def szp_decompress(fastMode, dataType, bytes_, byteLength, nbEle, blockSize):
+    """Mirror of szp_decompress (SZp/include/szp.h:36): returns float32[nbEle]."""
+    s = np.ascontiguousarray(np.frombuffer(bytes_, dtype=np.uint8) if isinstance(bytes_, (bytes, bytearray)) else bytes_,
+                             dtype=np.uint8)
+    p = lib.szp_decompress(int(fastMode), int(dataType), s.ctypes.data, int(byteLength),
+                           int(nbEle), int(blockSize))
+    if not p:
+        raise SzpError("szp_decompress failed: " + _last_error())
+    try:
+        n = int(nbEle)
+        out = (np.ctypeslib.as_array((ctypes.c_float * n).from_address(p)).copy() if n
+               else np.zeros(0, np.float32))
+    finally:
+        _libc.free(p)
+    return out
+
+
+class DeviceCodec:
+    """Device-resident compress/decompress on torch CUDA(HIP) tensors.
+
+    Holds the output and workspace tensors of the last shape it served, so
+    repeated calls on one field allocate nothing.  Calls are asynchronous on
+    the given torch stream (default: the current stream); `check()` waits and
+    raises on a device-side error.
+    """
+
+    def __init__(self, block_size=128, threads=None, device=None):
+        import torch
+        self.torch = torch
+        self.B = int(block_size)
+        self.T = int(threads) if threads else default_threads()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self._bufs = {}
+        self.last_work = None
+
+    def _buf(self, key, nbytes, dtype=None):
+        t = self._bufs.get(key)
+        torch = self.torch
+        if t is None or t.numel() < nbytes:
+            t = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=self.device)
+            self._bufs[key] = t
+        return t
+
+    def _stream(self, stream):
+        torch = self.torch
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def compress(self, x, err_mode=REL, abs_eb=0.0, rel=1e-4, out=None, stream=None):
+        """Returns (stream_tensor uint8[capacity], size_tensor int64[1]) ; the
+        valid stream is stream_tensor[:size]."""
+        torch = self.torch
+        assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous()
+        n = x.numel()
+        cap = compress_bound(n, self.B, self.T)
+        wsb = lib.szp_amd_compress_workspace_size(n, self.B, self.T)
+        if out is None:
+            out = self._buf("cout", cap)
+        size = self._buf("csize", 256).view(torch.int64)
+        work = self._buf("cwork", wsb)
+        self.last_work = work
+        rc = lib.szp_amd_compress_dev(x.data_ptr(), n, int(err_mode), float(abs_eb), float(rel),
+                                      self.B, self.T, out.data_ptr(), out.numel(), size.data_ptr(),
+                                      work.data_ptr(), work.numel(), self._stream(stream))
+        if rc:
+            raise SzpError("szp_amd_compress_dev", rc)
+        return out, size[:1]
+
+    def decompress(self, stream_t, stream_bytes, n, out=None, stream=None):
+        torch = self.torch
+        assert stream_t.is_cuda and stream_t.dtype == torch.uint8
+        wsb = lib.szp_amd_decompress_workspace_size(int(stream_bytes), int(n), self.B, self.T)
+        work = self._buf("dwork", wsb)
+        self.last_work = work
+        if out is None:
+            out = torch.empty(int(n), dtype=torch.float32, device=self.device)
+        rc = lib.szp_amd_decompress_dev(stream_t.data_ptr(), int(stream_bytes), int(n), self.B,
+                                        self.T, out.data_ptr(), work.data_ptr(), work.numel(),
+                                        self._stream(stream))
+        if rc:
+            raise SzpError("szp_amd_decompress_dev", rc)
+        return out
+
+    def check(self, stream=None):
+        if self.last_work is None:
+            return
+        rc = lib.szp_amd_check_status(self.last_work.data_ptr(), self._stream(stream))
+        if rc:
+            raise SzpError("device status", rc)

@@ -1,0 +1,413 @@
+// C-ABI of libszp_amd.so: the reference's fastmode1 float32 entry points
+// (include/szp.h) on host pointers, and the device-resident API
+// (include/szp_amd.h).  The host-pointer entry points copy through cached
+// device buffers and run the same kernels as the device API.
+#include <hip/hip_runtime.h>
+
+#include <dlfcn.h>
+#include <sched.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "szp.h"
+#include "szp_amd.h"
+#include "szp_internal.h"
+
+using namespace szp_amd;
+
+extern "C" {
+// SZp/src/szp.cc:17-20.  The stream's eb is always written big-endian, as
+// the reference does on little-endian hosts.
+int szp_versionNumber[4] = {1, 0, 0, 0};
+int szp_dataEndianType = LITTLE_ENDIAN_DATA;
+int szp_sysEndianType = LITTLE_ENDIAN_SYSTEM;
+}
+
+namespace {
+
+thread_local std::string t_last_error;
+int g_threads_override = 0;
+
+void set_error(const char *msg, int code = 0)
+{
+    t_last_error = msg;
+    if (code) {
+        t_last_error += ": ";
+        t_last_error += szp_amd_status_string(code);
+    }
+}
+
+int map_device_errors(uint32_t bits)
+{
+    if (bits & kErrCapacity) return SZP_AMD_ERR_CAPACITY;
+    if (bits & kErrStream) return SZP_AMD_ERR_STREAM;
+    if (bits & kErrTimeout) return SZP_AMD_ERR_TIMEOUT;
+    return SZP_AMD_OK;
+}
+
+// Device buffers reused by the host-pointer entry points of one device.
+struct Session {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t cap[4] = {0, 0, 0, 0};
+
+    void *ensure(int slot, size_t bytes)
+    {
+        if (bytes == 0) bytes = 256;
+        if (cap[slot] >= bytes) return buf[slot];
+        if (buf[slot]) (void)hipFree(buf[slot]);
+        buf[slot] = nullptr;
+        cap[slot] = 0;
+        if (hipMalloc(&buf[slot], bytes) != hipSuccess) return nullptr;
+        cap[slot] = bytes;
+        return buf[slot];
+    }
+};
+
+Session *session_for_current_device()
+{
+    static Session sessions[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    Session *s = &sessions[dev];
+    if (!s->stream) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->stream && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+            return nullptr;
+    }
+    return s;
+}
+
+uint64_t read_u64(const unsigned char *p)
+{
+    uint64_t v;
+    memcpy(&v, p, 8);
+    return v;
+}
+
+// Length of the part of a stream after the 4-byte eb, from the chunk offsets
+// and a walk over the last chunk's width bytes; 0 when malformed.
+uint64_t body_length(const unsigned char *body, uint64_t n, int B, int T)
+{
+    Geom g;
+    if (!make_geom(n, B, T, &g)) return 0;
+    const uint64_t hdr = 8ull * T;
+    if (n == 0) return hdr;
+    uint64_t pos = hdr + read_u64(body + hdr - 8) + 4;
+    const uint64_t nl = n - (uint64_t)(T - 1) * g.tbs;
+    for (uint64_t k = 0; k < g.nblast; ++k) {
+        const uint32_t b = body[pos];
+        if (b > 32) return 0;
+        const uint64_t L = (k + 1 == g.nblast) ? nl - 1 - k * (uint64_t)B : (uint64_t)B;
+        if (!b) {
+            pos += 1;
+        } else {
+            pos += 1 + (L + 7) / 8 + (uint64_t)(b / 8) * L + ((uint64_t)(b % 8) * L + 7) / 8;
+        }
+    }
+    return pos;
+}
+
+// Compress host data through the device; returns 0 and fills *stream (malloc'd
+// unless `into` is given) or a status code.
+int host_compress(const float *data, uint64_t n, int mode, double abs_eb, double rel, int B, int T,
+                  unsigned char *into, unsigned char **stream, size_t *out_size)
+{
+    Session *s = session_for_current_device();
+    if (!s) return SZP_AMD_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const uint64_t cap = compress_bound(n, B, T);
+    const uint64_t wsb = compress_workspace_bytes(n, B, T);
+    if (!cap || !wsb) return SZP_AMD_ERR_ARG;
+    float *d_in = (float *)s->ensure(0, n * 4);
+    unsigned char *d_out = (unsigned char *)s->ensure(1, cap);
+    void *d_ws = s->ensure(2, wsb);
+    uint64_t *d_size = (uint64_t *)s->ensure(3, 256);
+    if (!d_in || !d_out || !d_ws || !d_size) return SZP_AMD_ERR_NOMEM;
+    if (n && hipMemcpyAsync(d_in, data, n * 4, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return SZP_AMD_ERR_HIP;
+    int rc = launch_compress(d_in, n, mode, abs_eb, rel, B, T, d_out, cap, d_size, d_ws, wsb,
+                             s->stream);
+    if (rc) return rc;
+    rc = szp_amd_check_status(d_ws, s->stream);
+    if (rc) return rc;
+    uint64_t size = 0;
+    if (hipMemcpy(&size, d_size, 8, hipMemcpyDeviceToHost) != hipSuccess) return SZP_AMD_ERR_HIP;
+    unsigned char *dst = into ? into : (unsigned char *)malloc(size ? size : 1);
+    if (!dst) return SZP_AMD_ERR_NOMEM;
+    if (hipMemcpy(dst, d_out, size, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (!into) free(dst);
+        return SZP_AMD_ERR_HIP;
+    }
+    *stream = dst;
+    *out_size = size;
+    return SZP_AMD_OK;
+}
+
+// Decompress a host stream given as (eb bytes, body) into `into` or a new
+// malloc'd array.
+int host_decompress(const unsigned char eb_be[4], const unsigned char *body, uint64_t n, int B,
+                    int T, float *into, float **result)
+{
+    const uint64_t blen = body_length(body, n, B, T);
+    if (!blen) return SZP_AMD_ERR_STREAM;
+    const uint64_t len = 4 + blen;
+    Session *s = session_for_current_device();
+    if (!s) return SZP_AMD_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const uint64_t wsb = decompress_workspace_bytes(len, n, B, T);
+    if (!wsb) return SZP_AMD_ERR_ARG;
+    unsigned char *d_st = (unsigned char *)s->ensure(1, len);
+    float *d_out = (float *)s->ensure(0, n * 4);
+    void *d_ws = s->ensure(2, wsb);
+    if (!d_st || !d_out || !d_ws) return SZP_AMD_ERR_NOMEM;
+    if (hipMemcpyAsync(d_st, eb_be, 4, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(d_st + 4, body, blen, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return SZP_AMD_ERR_HIP;
+    int rc = launch_decompress(d_st, len, n, B, T, d_out, d_ws, wsb, s->stream);
+    if (rc) return rc;
+    rc = szp_amd_check_status(d_ws, s->stream);
+    if (rc) return rc;
+    float *dst = into ? into : (float *)malloc(n ? n * 4 : 4);
+    if (!dst) return SZP_AMD_ERR_NOMEM;
+    if (n && hipMemcpy(dst, d_out, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (!into) free(dst);
+        return SZP_AMD_ERR_HIP;
+    }
+    *result = dst;
+    return SZP_AMD_OK;
+}
+
+void float_to_be(float v, unsigned char b[4])
+{
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    b[0] = (unsigned char)(u >> 24);
+    b[1] = (unsigned char)(u >> 16);
+    b[2] = (unsigned char)(u >> 8);
+    b[3] = (unsigned char)u;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *szp_amd_status_string(int status)
+{
+    switch (status) {
+    case SZP_AMD_OK: return "ok";
+    case SZP_AMD_ERR_ARG: return "invalid argument";
+    case SZP_AMD_ERR_CAPACITY: return "output or workspace capacity too small";
+    case SZP_AMD_ERR_HIP: return "HIP runtime error";
+    case SZP_AMD_ERR_STREAM: return "malformed compressed stream";
+    case SZP_AMD_ERR_TIMEOUT: return "device wait timed out";
+    case SZP_AMD_ERR_UNSUPPORTED: return "only fastMode 1 (SZP_NONRANDOMACCESS) float32 is supported";
+    case SZP_AMD_ERR_NOMEM: return "allocation failed";
+    default: return "unknown status";
+    }
+}
+
+const char *szp_amd_last_error(void) { return t_last_error.c_str(); }
+
+int szp_amd_default_threads(void)
+{
+    if (g_threads_override > 0) return g_threads_override;
+    const char *e = getenv("SZP_AMD_NUM_THREADS");
+    if (e && atoi(e) > 0) return atoi(e);
+    typedef int (*omp_fn)(void);
+    omp_fn f = (omp_fn)dlsym(RTLD_DEFAULT, "omp_get_max_threads");
+    if (f) {
+        int v = f();
+        if (v > 0) return v;
+    }
+    e = getenv("OMP_NUM_THREADS");
+    if (e && atoi(e) > 0) return atoi(e);
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0 && CPU_COUNT(&set) > 0) return CPU_COUNT(&set);
+    return 1;
+}
+
+void szp_amd_set_threads(int nbThreads) { g_threads_override = nbThreads > 0 ? nbThreads : 0; }
+
+int szp_amd_bit_width(uint32_t maxMagnitude) { return bit_width_host(maxMagnitude); }
+
+uint64_t szp_amd_compress_bound(uint64_t nbEle, int blockSize, int nbThreads)
+{
+    return compress_bound(nbEle, blockSize, nbThreads);
+}
+
+uint64_t szp_amd_compress_workspace_size(uint64_t nbEle, int blockSize, int nbThreads)
+{
+    return compress_workspace_bytes(nbEle, blockSize, nbThreads);
+}
+
+uint64_t szp_amd_decompress_workspace_size(uint64_t streamBytes, uint64_t nbEle, int blockSize,
+                                           int nbThreads)
+{
+    return decompress_workspace_bytes(streamBytes, nbEle, blockSize, nbThreads);
+}
+
+int szp_amd_compress_dev(const float *d_data, uint64_t nbEle, int errBoundMode,
+                         double absErrBound, double relBoundRatio, int blockSize, int nbThreads,
+                         unsigned char *d_out, uint64_t outCapacity, uint64_t *d_outSize,
+                         void *d_work, uint64_t workBytes, void *stream)
+{
+    return launch_compress(d_data, nbEle, errBoundMode, absErrBound, relBoundRatio, blockSize,
+                           nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
+                           (hipStream_t)stream);
+}
+
+int szp_amd_decompress_dev(const unsigned char *d_stream, uint64_t streamBytes, uint64_t nbEle,
+                           int blockSize, int nbThreads, float *d_out, void *d_work,
+                           uint64_t workBytes, void *stream)
+{
+    return launch_decompress(d_stream, streamBytes, nbEle, blockSize, nbThreads, d_out, d_work,
+                             workBytes, (hipStream_t)stream);
+}
+
+int szp_amd_check_status(void *d_work, void *stream)
+{
+    if (!d_work) return SZP_AMD_ERR_ARG;
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return SZP_AMD_ERR_HIP;
+    uint32_t bits = 0;
+    if (hipMemcpy(&bits, workspace_error_word(d_work), 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return SZP_AMD_ERR_HIP;
+    return map_device_errors(bits);
+}
+
+uint64_t szp_amd_stream_length(const unsigned char *bytes, uint64_t nbEle, int blockSize,
+                               int nbThreads)
+{
+    if (!bytes) return 0;
+    const uint64_t b = body_length(bytes + 4, nbEle, blockSize, nbThreads);
+    return b ? 4 + b : 0;
+}
+
+unsigned char *szp_compress(int fastMode, int dataType, void *data, size_t *outSize,
+                            int errBoundMode, double absErrBound, double relBoundRatio,
+                            size_t nbEle, int blockSize)
+{
+    if (fastMode != SZP_NONRANDOMACCESS || dataType != SZ_FLOAT) {
+        set_error("szp_compress", SZP_AMD_ERR_UNSUPPORTED);
+        return NULL;
+    }
+    if (!outSize || (!data && nbEle)) {
+        set_error("szp_compress", SZP_AMD_ERR_ARG);
+        return NULL;
+    }
+    unsigned char *stream = NULL;
+    size_t size = 0;
+    const int rc = host_compress((const float *)data, nbEle, errBoundMode, absErrBound,
+                                 relBoundRatio, blockSize, szp_amd_default_threads(), NULL,
+                                 &stream, &size);
+    if (rc) {
+        set_error("szp_compress", rc);
+        return NULL;
+    }
+    *outSize = size;
+    return stream;
+}
+
+void *szp_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
+                     size_t nbEle, int blockSize)
+{
+    (void)byteLength;  // unused by the reference too (SZp/src/szp.cc:118-153)
+    if (fastMode != SZP_NONRANDOMACCESS || dataType != SZ_FLOAT) {
+        set_error("szp_decompress", SZP_AMD_ERR_UNSUPPORTED);
+        return NULL;
+    }
+    if (!bytes) {
+        set_error("szp_decompress", SZP_AMD_ERR_ARG);
+        return NULL;
+    }
+    szp_sysEndianType = LITTLE_ENDIAN_SYSTEM;  // SZp/src/szp.cc:120-125 on x86-64
+    float *out = NULL;
+    const int rc = host_decompress(bytes, bytes + 4, nbEle, blockSize, szp_amd_default_threads(),
+                                   NULL, &out);
+    if (rc) {
+        set_error("szp_decompress", rc);
+        return NULL;
+    }
+    return out;
+}
+
+unsigned char *szp_float_openmp_threadblock(float *oriData, size_t *outSize, float absErrBound,
+                                            size_t nbEle, int blockSize)
+{
+    if (!outSize || (!oriData && nbEle)) {
+        set_error("szp_float_openmp_threadblock", SZP_AMD_ERR_ARG);
+        return NULL;
+    }
+    unsigned char *stream = NULL;
+    size_t size = 0;
+    const int rc = host_compress(oriData, nbEle, ABS, (double)absErrBound, 0.0, blockSize,
+                                 szp_amd_default_threads(), NULL, &stream, &size);
+    if (rc) {
+        set_error("szp_float_openmp_threadblock", rc);
+        return NULL;
+    }
+    *outSize = size;
+    return stream;
+}
+
+void szp_float_openmp_threadblock_arg(unsigned char *output, float *oriData, size_t *outSize,
+                                      float absErrBound, size_t nbEle, int blockSize)
+{
+    if (!output || !outSize || (!oriData && nbEle)) {
+        set_error("szp_float_openmp_threadblock_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    unsigned char *stream = NULL;
+    size_t size = 0;
+    const int rc = host_compress(oriData, nbEle, ABS, (double)absErrBound, 0.0, blockSize,
+                                 szp_amd_default_threads(), output, &stream, &size);
+    if (rc) {
+        set_error("szp_float_openmp_threadblock_arg", rc);
+        *outSize = 0;
+        return;
+    }
+    *outSize = size;
+}
+
+float *szp_float_decompress_openmp_threadblock(size_t nbEle, float absErrBound, int blockSize,
+                                               unsigned char *cmpBytes)
+{
+    if (!cmpBytes) {
+        set_error("szp_float_decompress_openmp_threadblock", SZP_AMD_ERR_ARG);
+        return NULL;
+    }
+    unsigned char eb[4];
+    float_to_be(absErrBound, eb);
+    float *out = NULL;
+    const int rc = host_decompress(eb, cmpBytes, nbEle, blockSize, szp_amd_default_threads(), NULL,
+                                   &out);
+    if (rc) {
+        set_error("szp_float_decompress_openmp_threadblock", rc);
+        return NULL;
+    }
+    return out;
+}
+
+void szp_float_decompress_openmp_threadblock_arg(float *newData, size_t nbEle, float absErrBound,
+                                                 int blockSize, unsigned char *cmpBytes)
+{
+    if (!cmpBytes || (!newData && nbEle)) {
+        set_error("szp_float_decompress_openmp_threadblock_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    unsigned char eb[4];
+    float_to_be(absErrBound, eb);
+    float *out = NULL;
+    const int rc = host_decompress(eb, cmpBytes, nbEle, blockSize, szp_amd_default_threads(),
+                                   newData, &out);
+    if (rc) set_error("szp_float_decompress_openmp_threadblock_arg", rc);
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+"""ctypes access to the CPU oracle (oracle/libszp_oracle.so) and, when built,
+the reference compiled from its own sources (oracle/_ref/libszp_ref.so).
+
+Test infrastructure only: the oracle is the checker, never the thing tested.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "libszp_oracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libszp_ref.so")
+
+_O = None
+_R = None
+
+
+def oracle():
+    global _O
+    if _O is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        O = ctypes.CDLL(ORACLE_SO)
+        u64, i32, vp, f32, dbl = (ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_float,
+                                  ctypes.c_double)
+        O.szpo_bit_width.argtypes = [ctypes.c_uint32]
+        O.szpo_bit_width_table.argtypes = [ctypes.c_uint32]
+        O.szpo_check_bit_width_range.restype = u64
+        O.szpo_check_bit_width_range.argtypes = [u64, u64, ctypes.POINTER(ctypes.c_uint32)]
+        O.szpo_inverse_bound.restype = f32
+        O.szpo_inverse_bound.argtypes = [f32]
+        O.szpo_quantize.restype = ctypes.c_int32
+        O.szpo_quantize.argtypes = [f32, f32]
+        O.szpo_block_bytes.restype = u64
+        O.szpo_block_bytes.argtypes = [i32, u64]
+        O.szpo_encode_block.restype = u64
+        O.szpo_encode_block.argtypes = [vp, u64, vp]
+        O.szpo_decode_block.restype = u64
+        O.szpo_decode_block.argtypes = [vp, u64, vp]
+        O.szpo_rel_bound.restype = f32
+        O.szpo_rel_bound.argtypes = [vp, u64, dbl]
+        O.szpo_compress_bound.restype = u64
+        O.szpo_compress_bound.argtypes = [u64, i32, i32]
+        O.szpo_compress.argtypes = [vp, u64, f32, i32, i32, i32, vp, u64, ctypes.POINTER(u64)]
+        O.szpo_decompress.argtypes = [vp, u64, u64, i32, i32, i32, vp]
+        O.szpo_compress_api.argtypes = [vp, u64, i32, dbl, dbl, i32, i32, i32, vp, u64,
+                                        ctypes.POINTER(u64)]
+        _O = O
+    return _O
+
+
+def compress(x, mode, abs_eb, rel, B, T, workers=0):
+    """Oracle stream (uint8 array) for szp_compress(1, SZ_FLOAT, x, mode, ...) with team size T."""
+    O = oracle()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    cap = O.szpo_compress_bound(x.size, B, T)
+    out = np.empty(cap, np.uint8)
+    sz = ctypes.c_uint64()
+    rc = O.szpo_compress_api(x.ctypes.data, x.size, mode, abs_eb, rel, B, T, workers,
+                             out.ctypes.data, cap, ctypes.byref(sz))
+    if rc:
+        raise ValueError("oracle compress rejected the arguments")
+    return out[:sz.value].copy()
+
+
+def decompress(stream, n, B, T, workers=0):
+    O = oracle()
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = np.empty(n, np.float32)
+    rc = O.szpo_decompress(s.ctypes.data, s.size, n, B, T, workers, out.ctypes.data)
+    if rc:
+        raise ValueError("oracle decompress found a malformed stream")
+    return out
+
+
+def reference():
+    """The reference library compiled from its sources, or None when not built."""
+    global _R
+    if _R is None and os.path.exists(REF_SO):
+        R = ctypes.CDLL(REF_SO)
+        R.szp_compress.restype = ctypes.c_void_p
+        R.szp_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_size_t), ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_size_t, ctypes.c_int]
+        R.szp_decompress.restype = ctypes.c_void_p
+        R.szp_decompress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_size_t, ctypes.c_int]
+        R.szp_ref_set_threads.argtypes = [ctypes.c_int]
+        _R = R
+    return _R
+
+
+_libc = ctypes.CDLL("libc.so.6")
+_libc.free.argtypes = [ctypes.c_void_p]
+
+
+def ref_fits(stream, n, T):
+    """True when the reference's fixed buffers (4N+4 total, ceil(4N/T) per
+    chunk, SZp/src/szp_float.cc:156,186) hold this stream."""
+    hdr = 4 + 8 * T
+    offs = np.frombuffer(stream[4:hdr].tobytes(), np.uint64).astype(np.int64)
+    sizes = np.diff(np.append(offs, stream.size - hdr))
+    return stream.size <= 4 * n + 4 and sizes.max() <= (4 * n + T - 1) // T
+
+
+def ref_compress(x, mode, abs_eb, rel, B, T):
+    R = reference()
+    R.szp_ref_set_threads(T)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    sz = ctypes.c_size_t()
+    p = R.szp_compress(1, 0, x.ctypes.data, ctypes.byref(sz), mode, abs_eb, rel, x.size, B)
+    out = np.ctypeslib.as_array((ctypes.c_uint8 * sz.value).from_address(p)).copy()
+    _libc.free(p)
+    return out
+
+
+def ref_decompress(stream, n, B, T):
+    R = reference()
+    R.szp_ref_set_threads(T)
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    p = R.szp_decompress(1, 0, s.ctypes.data, s.size, n, B)
+    out = np.ctypeslib.as_array((ctypes.c_float * n).from_address(p)).copy()
+    _libc.free(p)
+    return out
