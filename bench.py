@@ -1,0 +1,273 @@
+#!/usr/bin/env python3
+"""SZp fastmode1 float32 compress + decompress on one MI355X.
+
+Workload: a 512^3 float32 Gaussian random field with isotropic power spectrum
+proportional to k^-3, normalised to [-1, 1] (synthetic, seeded), compressed
+with a relative error bound of 1e-4 and block size 128 into T = 16 chunks.
+One step = one compress of the field plus one decompress of its stream, with
+the field already resident in HBM.  Variants (REL 1e-2, B = 32) are measured
+after the main line and reported beside it.
+
+value = 2 * 4N / step time: uncompressed bytes through the codec per second
+(compress consumes 4N bytes, decompress produces 4N bytes).
+
+The stream is checked byte-for-byte, and the decompression bit-for-bit,
+against the reference compiled from its own sources (oracle/_ref) run on the
+same field with the same T; the reference's OpenMP path on the host cores is
+also timed as the CPU baseline.  Prints one JSON line.
+"""
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_grf(torch, n, seed):
+    """512^3 k^-3 Gaussian random field in [-1, 1], built on the GPU."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    w = torch.randn((n, n, n), generator=g, device="cuda", dtype=torch.float32)
+    F = torch.fft.rfftn(w)
+    del w
+    kx = torch.fft.fftfreq(n, device="cuda").view(n, 1, 1)
+    ky = torch.fft.fftfreq(n, device="cuda").view(1, n, 1)
+    kz = torch.fft.rfftfreq(n, device="cuda").view(1, 1, n // 2 + 1)
+    k = torch.sqrt(kx * kx + ky * ky + kz * kz)
+    k[0, 0, 0] = 1.0
+    F *= k.pow(-1.5)
+    F[0, 0, 0] = 0
+    del k
+    x = torch.fft.irfftn(F, s=(n, n, n))
+    del F
+    mn, mx = x.min(), x.max()
+    x = ((x - mn) / (mx - mn) * 2 - 1).contiguous().view(-1)
+    torch.cuda.synchronize()
+    return x
+
+
+def load_reference(threads):
+    """The reference library compiled from its own sources, else the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    if oracle_lib.reference() is not None:
+        kind = "reference"
+
+        def comp(x, rel, B):
+            return oracle_lib.ref_compress(x, 1, 0.0, rel, B, threads)
+
+        def decomp(s, n, B):
+            return oracle_lib.ref_decompress(s, n, B, threads)
+    else:
+        kind = "port"
+
+        def comp(x, rel, B):
+            return oracle_lib.compress(x, 1, 0.0, rel, B, threads, threads)
+
+        def decomp(s, n, B):
+            return oracle_lib.decompress(s, n, B, threads, threads)
+    return kind, comp, decomp
+
+
+def measure(torch, szp, codec, x, rel, steps, warmup, profile):
+    n = x.numel()
+    out, size_t = codec.compress(x, szp.REL, 0.0, rel)
+    codec.check()
+    nbytes = int(size_t.item())
+    dec = torch.empty(n, dtype=torch.float32, device=x.device)
+    for _ in range(warmup):
+        codec.compress(x, szp.REL, 0.0, rel, out=out)
+        codec.decompress(out, nbytes, n, out=dec)
+    codec.check()
+    if int(size_t.item()) != nbytes:
+        raise RuntimeError("compressed size changed between runs")
+
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    if profile:
+        szp.profile_enable(True)
+    e0.record()
+    for _ in range(steps):
+        codec.compress(x, szp.REL, 0.0, rel, out=out)
+        codec.decompress(out, nbytes, n, out=dec)
+    e1.record()
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1) / steps
+    kernels = szp.profile_read() if profile else {}
+    if profile:
+        szp.profile_enable(False)
+
+    e0.record()
+    for _ in range(steps):
+        codec.compress(x, szp.REL, 0.0, rel, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    comp_ms = e0.elapsed_time(e1) / steps
+    e0.record()
+    for _ in range(steps):
+        codec.decompress(out, nbytes, n, out=dec)
+    e1.record()
+    torch.cuda.synchronize()
+    dec_ms = e0.elapsed_time(e1) / steps
+    codec.check()
+    return dict(out=out, nbytes=nbytes, dec=dec, step_ms=step_ms, comp_ms=comp_ms, dec_ms=dec_ms,
+                kernels=kernels)
+
+
+def eb_report(torch, x, dec, stream_head):
+    eb = float(torch.frombuffer(bytearray(stream_head[:4][::-1]), dtype=torch.float32)[0])
+    err = (x.double() - dec.double()).abs()
+    tol = eb + 2.0 ** -22 * (eb + dec.double().abs())
+    return dict(eb=eb, max_err_over_eb=float(err.max()) / eb,
+                strict_over_eb=int((err > eb).sum()), within_rounding_bound=bool((err <= tol).all()))
+
+
+def traffic_for(kernel, workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    e = t.get(workload, {}).get(kernel)
+    return e.get("hbm_bytes_per_launch") if e else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512, help="field edge length")
+    ap.add_argument("--rel", type=float, default=1e-4)
+    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--threads", type=int, default=16, help="T, the reference's OpenMP team size")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the codec would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import szp_amd as szp
+
+    n3, B, T, rel = args.n, args.block, args.threads, args.rel
+    log("generating %d^3 k^-3 field" % n3)
+    x = make_grf(torch, n3, args.seed)
+    N = x.numel()
+    codec = szp.DeviceCodec(block_size=B, threads=T)
+    log("main: REL %g B=%d T=%d" % (rel, B, T))
+    r = measure(torch, szp, codec, x, rel, args.steps, args.warmup, profile=True)
+    C = r["nbytes"]
+    stream_host = r["out"][:C].cpu().numpy()
+    rep = eb_report(torch, x, r["dec"], stream_host)
+
+    # Dominant kernel and its roofline (HBM-bound integer work; no MFMA).
+    alg = {"k_compress": 4 * N + C, "k_decode": C + 4 * N, "k_parse": C, "k_range_partial": 4 * N}
+    kern = {k: v[1] / v[0] for k, v in r["kernels"].items() if v[0]}
+    dom = max((k for k in kern if k in alg), key=lambda k: kern[k])
+    achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
+    workload = "grf512_k3_rel%g_B%d_T%d" % (rel, B, T)
+    roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBPS, 4),
+                    traffic=traffic_for(dom, workload), alg_bytes_per_launch=alg[dom],
+                    avg_launch_ms=round(kern[dom], 5),
+                    kernels_avg_ms={k: round(v, 5) for k, v in sorted(kern.items())})
+
+    comp_gbps = 4 * N / (r["comp_ms"] * 1e-3) / 1e9
+    dec_gbps = 4 * N / (r["dec_ms"] * 1e-3) / 1e9
+    value = 8 * N / (r["step_ms"] * 1e-3) / 1e9
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        kind, rcomp, rdecomp = load_reference(T)
+        xh = x.cpu().numpy()
+        log("cpu %s: compress + decompress on %d threads" % (kind, T))
+        ts_c, ts_d = [], []
+        ref_stream = ref_dec = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref_stream = rcomp(xh, rel, B)
+            t1 = time.perf_counter()
+            ref_dec = rdecomp(ref_stream, N, B)
+            t2 = time.perf_counter()
+            ts_c.append(t1 - t0)
+            ts_d.append(t2 - t1)
+        tc, td = statistics.median(ts_c), statistics.median(ts_d)
+        cpu = dict(value=round(8 * N / (tc + td) / 1e9, 3), unit="GB/s", cores=T, kind=kind,
+                   sample="the full 512^3 field, REL %g, B=%d: median of 3 (compress %.3f s, "
+                          "decompress %.3f s)" % (rel, B, tc, td),
+                   compress_GBps=round(4 * N / tc / 1e9, 3),
+                   decompress_GBps=round(4 * N / td / 1e9, 3))
+        gpu_dec = r["dec"].cpu().numpy()
+        parity = dict(stream_byte_identical=bool(ref_stream.size == C and
+                                                 np.array_equal(ref_stream, stream_host)),
+                      decompress_bit_identical=bool(np.array_equal(ref_dec.view(np.uint32),
+                                                                   gpu_dec.view(np.uint32))),
+                      against=kind)
+
+    variants = []
+    if not args.no_variants:
+        for vrel, vB in ((1e-2, 128), (1e-4, 32), (1e-2, 32)):
+            log("variant: REL %g B=%d" % (vrel, vB))
+            vc = szp.DeviceCodec(block_size=vB, threads=T)
+            vr = measure(torch, szp, vc, x, vrel, max(5, args.steps // 2), 2, profile=False)
+            v = dict(rel=vrel, B=vB, ratio=round(4 * N / vr["nbytes"], 4),
+                     compress_GBps=round(4 * N / (vr["comp_ms"] * 1e-3) / 1e9, 1),
+                     decompress_GBps=round(4 * N / (vr["dec_ms"] * 1e-3) / 1e9, 1),
+                     compress_hbm_frac=round((8 * N + vr["nbytes"]) / (vr["comp_ms"] * 1e-3) / 8e12, 4),
+                     decompress_hbm_frac=round((vr["nbytes"] + 4 * N) / (vr["dec_ms"] * 1e-3) / 8e12, 4))
+            if not args.no_cpu:
+                vs = vr["out"][:vr["nbytes"]].cpu().numpy()
+                rs = rcomp(xh, vrel, vB)
+                v["stream_byte_identical"] = bool(rs.size == vs.size and np.array_equal(rs, vs))
+            variants.append(v)
+            del vc, vr
+
+    line = {
+        "metric": "compress/decompress GB/s and %HBM peak, 1 GPU",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["step_ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "512^3 float32 k^-3 Gaussian random field in [-1,1], SZp fastmode1, "
+                               "REL %g, block %d, T=%d" % (rel, B, T),
+                   "n_elements": N, "field_bytes": 4 * N, "error_bound_mode": "REL",
+                   "rel": rel, "block_size": B, "threads_T": T},
+        "compress_GBps": round(comp_gbps, 1),
+        "decompress_GBps": round(dec_gbps, 1),
+        "compress_hbm_frac": round((8 * N + C) / (r["comp_ms"] * 1e-3) / 8e12, 4),
+        "decompress_hbm_frac": round((C + 4 * N) / (r["dec_ms"] * 1e-3) / 8e12, 4),
+        "compressed_bytes": C,
+        "ratio": round(4 * N / C, 4),
+        "error_bound": rep,
+        "parity": parity,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -75,6 +75,16 @@ void szp_amd_set_threads(int nbThreads);
  * (int)log2f((float)max) + 1 (SZp/src/szp_float.cc:253-262). */
 int szp_amd_bit_width(uint32_t maxMagnitude);
 
+/* Per-kernel device timing, the device counterpart of the reference's
+ * memory-access-count instrumentation (SZp/src/szp_float.cc:607-750).  While
+ * enabled, each kernel launch is bracketed by HIP events on its stream;
+ * reading syncs those events and returns the per-kernel launch count and
+ * summed milliseconds.  enable(1) and enable(0) both clear the totals. */
+void szp_amd_profile_enable(int enable);
+int szp_amd_profile_count(void);
+int szp_amd_profile_entry(int index, char *name, int nameCapacity, uint64_t *launches,
+                          double *totalMs);
+
 const char *szp_amd_status_string(int status);
 /* Message of the last failure of a szp.h entry point in this thread. */
 const char *szp_amd_last_error(void);

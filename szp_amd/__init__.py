@@ -20,7 +20,7 @@ __all__ = [
     "ABS", "REL", "SZP_NONRANDOMACCESS", "SZP_RANDOMACCESS", "SZ_FLOAT", "SZ_DOUBLE",
     "SzpError", "lib", "library_path", "szp_compress", "szp_decompress",
     "compress_bound", "default_threads", "set_threads", "bit_width", "stream_length",
-    "DeviceCodec",
+    "DeviceCodec", "profile_enable", "profile_read",
 ]
 
 # SZp/include/szp_defines.h:19-20,33-37
@@ -50,7 +50,7 @@ class SzpError(RuntimeError):
 def _load():
     if not os.path.exists(library_path):
         raise ImportError(
-            "libszp_amd.so is not built (%s); run `python -m szp_amd.build` or "
+            "libszp_amd.so is not built (%s); run `python szp_amd/build.py` or "
             "__graft_entry__.build()" % library_path)
     L = ctypes.CDLL(library_path)
     u64, i32, vp, dbl, f32 = (ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
@@ -75,6 +75,10 @@ def _load():
         "szp_amd_bit_width": (i32, [ctypes.c_uint32]),
         "szp_amd_status_string": (ctypes.c_char_p, [i32]),
         "szp_amd_last_error": (ctypes.c_char_p, []),
+        "szp_amd_profile_enable": (None, [i32]),
+        "szp_amd_profile_count": (i32, []),
+        "szp_amd_profile_entry": (i32, [i32, ctypes.c_char_p, i32, ctypes.POINTER(u64),
+                                        ctypes.POINTER(dbl)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -91,6 +95,23 @@ _libc.free.restype = None
 
 def _last_error():
     return (lib.szp_amd_last_error() or b"").decode()
+
+
+def profile_enable(on=True):
+    """Starts (or stops) per-kernel event timing; both clear the totals."""
+    lib.szp_amd_profile_enable(1 if on else 0)
+
+
+def profile_read():
+    """{kernel name: (launches, total ms)} since profile_enable(True)."""
+    out = {}
+    for i in range(lib.szp_amd_profile_count()):
+        name = ctypes.create_string_buffer(64)
+        n = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        if lib.szp_amd_profile_entry(i, name, 64, ctypes.byref(n), ctypes.byref(ms)) == 0:
+            out[name.value.decode()] = (n.value, ms.value)
+    return out
 
 
 def default_threads():
@@ -222,8 +243,11 @@ class DeviceCodec:
         return out
 
     def check(self, stream=None):
-        if self.last_work is None:
-            return
-        rc = lib.szp_amd_check_status(self.last_work.data_ptr(), self._stream(stream))
-        if rc:
-            raise SzpError("device status", rc)
+        """Waits for the stream and raises on an error recorded by the device."""
+        for key in ("cwork", "dwork"):
+            work = self._bufs.get(key)
+            if work is None:
+                continue
+            rc = lib.szp_amd_check_status(work.data_ptr(), self._stream(stream))
+            if rc:
+                raise SzpError("device status (%s)" % ("compress" if key == "cwork" else "decompress"), rc)

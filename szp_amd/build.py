@@ -1,6 +1,6 @@
 """Builds libszp_amd.so (HIP kernels + C ABI) in-tree for gfx950 with hipcc.
 
-    python -m szp_amd.build      # or __graft_entry__.build()
+    python szp_amd/build.py      # or __graft_entry__.build(); runs without importing the package
 """
 import os
 import subprocess

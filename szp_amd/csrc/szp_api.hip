@@ -238,6 +238,16 @@ void szp_amd_set_threads(int nbThreads) { g_threads_override = nbThreads > 0 ? n
 
 int szp_amd_bit_width(uint32_t maxMagnitude) { return bit_width_host(maxMagnitude); }
 
+void szp_amd_profile_enable(int enable) { profile_enable(enable); }
+
+int szp_amd_profile_count(void) { return profile_count(); }
+
+int szp_amd_profile_entry(int index, char *name, int nameCapacity, uint64_t *launches,
+                          double *totalMs)
+{
+    return profile_entry(index, name, nameCapacity, launches, totalMs);
+}
+
 uint64_t szp_amd_compress_bound(uint64_t nbEle, int blockSize, int nbThreads)
 {
     return compress_bound(nbEle, blockSize, nbThreads);

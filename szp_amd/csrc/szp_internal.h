@@ -8,7 +8,10 @@ namespace szp_amd {
 
 constexpr int kWG = 256;                 // threads per workgroup (4 wave64)
 constexpr int kMaxBlock = 2048;          // largest SZp block size handled on the GPU
-constexpr int kOutBufBytes = 8704 + 64;  // LDS bytes for one tile's encoded output
+constexpr int kOutBufBytes = 8704 + 64;  // LDS bytes for one round's encoded blocks
+constexpr int kMaxRounds = 8;            // rounds of K items per tile
+constexpr int kOutCap = 32768;           // LDS bytes buffering a compress tile before placement
+constexpr int kDecTileBytes = 24576;     // LDS bytes holding a decode tile's stream bytes
 
 // Geometry of the item sequence of one field.  The stream body is the
 // concatenation, in element order, of "items": the raw head value of each
@@ -29,7 +32,7 @@ struct Geom {
     uint32_t spb;      // slots per item
     uint32_t K;        // items per tile
     uint32_t spb_magic;  // slot / spb == (slot * spb_magic) >> 16 for slot < 256
-    uint32_t pad;
+    uint32_t R;          // rounds per tile (1 when chunks are smaller than a tile)
 };
 
 // Host-side construction; returns false when (n, B, T) is outside what the
@@ -58,6 +61,11 @@ int launch_decompress(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
 
 // Device address of the workspace's error word (u32).
 uint32_t *workspace_error_word(void *d_work);
+
+// Per-kernel event timing (off by default).
+void profile_enable(int enable);
+int profile_count();
+int profile_entry(int i, char *name, int cap, uint64_t *launches, double *total_ms);
 
 // Bit width of a block maximum exactly as the reference computes it
 // ((int)log2f((float)max) + 1), via clz and a threshold table (host copy).

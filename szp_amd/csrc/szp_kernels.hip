@@ -20,6 +20,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
 
 #include "szp_internal.h"
 
@@ -76,11 +83,14 @@ __host__ __device__ __forceinline__ uint32_t block_bytes(uint32_t b, uint32_t L)
 }
 
 // (int)((double)x * (double)inv) with x86 cvttsd2si semantics: NaN and
-// out-of-range products give INT32_MIN (SZp/src/szp_float.cc:209,228).
+// out-of-range products give INT32_MIN (SZp/src/szp_float.cc:209,228).  The
+// product of two floats is exact in double.  |p| < 2^31 admits every product
+// whose truncation fits int32 except (-2^31 - 1, -2^31], whose truncation is
+// INT32_MIN anyway; NaN fails the compare.
 __device__ __forceinline__ int32_t quantize(float x, float inv)
 {
-    double p = (double)x * (double)inv;
-    return (p > -2147483649.0 && p < 2147483648.0) ? (int32_t)p : INT32_MIN;
+    const double p = (double)x * (double)inv;
+    return __builtin_fabs(p) < 2147483648.0 ? (int32_t)p : INT32_MIN;
 }
 
 __device__ __forceinline__ uint64_t ld_status(const uint64_t *p)
@@ -400,7 +410,194 @@ __global__ void k_empty_stream(uint8_t *out, const float *d_eb, float eb_host, u
     if (threadIdx.x == 0) *d_size = 4 + 8ull * T;
 }
 
-template <bool VEC>
+// Unaligned LDS accesses (gfx950 runs with unaligned LDS access enabled; the
+// compiler emits ds_read/ds_write_b32/b64/b128 for these copies).
+__device__ __forceinline__ void lds_st32(uint8_t *p, uint32_t v) { __builtin_memcpy(p, &v, 4); }
+__device__ __forceinline__ void lds_st64(uint8_t *p, uint32_t lo, uint32_t hi)
+{
+    const uint2 v = make_uint2(lo, hi);
+    __builtin_memcpy(p, &v, 8);
+}
+__device__ __forceinline__ void lds_st128(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    const uint4 v = make_uint4(a, b, c, d);
+    __builtin_memcpy(p, &v, 16);
+}
+__device__ __forceinline__ uint2 lds_ld64(const uint8_t *p)
+{
+    uint2 v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+__device__ __forceinline__ uint4 lds_ld128(const uint8_t *p)
+{
+    uint4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+// Writes the n (<= 8) low-order bytes of w to p, little-endian.
+__device__ __forceinline__ void lds_st_bytes(uint8_t *p, uint64_t w, uint32_t n)
+{
+    if (n & 4) {
+        lds_st32(p, (uint32_t)w);
+        p += 4;
+        w >>= 32;
+    }
+    if (n & 2) {
+        const uint16_t v = (uint16_t)w;
+        __builtin_memcpy(p, &v, 2);
+        p += 2;
+        w >>= 16;
+    }
+    if (n & 1) *p = (uint8_t)w;
+}
+
+// One thread's octet of a round: up to 8 consecutive elements of one item.
+struct Oct {
+    uint64_t e0;  // first element
+    uint32_t m;   // elements in the octet (0: nothing to do)
+    uint32_t L;   // elements of the item
+    uint32_t t;   // chunk of the item
+    bool live;    // the slot's item exists in this round
+    bool head;    // the item is a chunk head
+};
+
+__device__ __forceinline__ Oct octet_of(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nit,
+                                        uint32_t d, uint32_t o)
+{
+    Oct s{0, 0, 0, 0, false, false};
+    if (d >= g.K || d >= nit) return s;
+    uint32_t t = ct;
+    uint64_t j = cj;
+    advance(g, t, j, d);
+    s.live = true;
+    s.t = t;
+    s.head = j == 0;
+    s.L = item_len(g, t, j);
+    const uint64_t es = item_elem(g, t, j);
+    if (s.head) {
+        s.e0 = es;
+        s.m = o == 0 ? 1u : 0u;
+    } else {
+        s.e0 = es + 8ull * o;
+        s.m = 8 * o < s.L ? min(8u, s.L - 8 * o) : 0u;
+    }
+    return s;
+}
+
+// v[0]: the element before the octet (blocks only); v[1..8]: the octet.
+struct OctIn {
+    float v[9];
+};
+
+__device__ __forceinline__ void octet_load(const float *__restrict__ x, const Oct &s, OctIn &in)
+{
+    if (s.m == 8) {
+        float4 a, b;
+        __builtin_memcpy(&a, x + s.e0, 16);
+        __builtin_memcpy(&b, x + s.e0 + 4, 16);
+        in.v[0] = x[s.e0 - 1];
+        in.v[1] = a.x; in.v[2] = a.y; in.v[3] = a.z; in.v[4] = a.w;
+        in.v[5] = b.x; in.v[6] = b.y; in.v[7] = b.z; in.v[8] = b.w;
+    } else {
+        in.v[0] = (s.m && !s.head) ? x[s.e0 - 1] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in.v[i + 1] = (uint32_t)i < s.m ? x[s.e0 + i] : 0.0f;
+    }
+}
+
+// Lorenzo deltas of the octet: magnitudes, MSB-first sign byte, maximum
+// (SZp/src/szp_float.cc:220-251); the head's raw q for a head slot.
+__device__ __forceinline__ uint32_t octet_deltas(const Oct &s, const OctIn &in, float inv,
+                                                 uint32_t mg[8], uint32_t &sgn, int32_t &qhead)
+{
+    sgn = 0;
+    qhead = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mg[i] = 0;
+    if (!s.m) return 0;
+    if (s.head) {
+        qhead = quantize(in.v[1], inv);
+        return 0;
+    }
+    int32_t prev = quantize(in.v[0], inv);
+    uint32_t omax = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int32_t c = quantize(in.v[i + 1], inv);
+        const uint32_t dd = (uint32_t)c - (uint32_t)prev;
+        prev = c;
+        const bool keep = (uint32_t)i < s.m;
+        const bool neg = keep && (int32_t)dd < 0;
+        const uint32_t mag = keep ? (neg ? 0u - dd : dd) : 0u;
+        mg[i] = mag;
+        sgn |= (uint32_t)neg << (7 - i);
+        omax = max(omax, mag);
+    }
+    return omax;
+}
+
+// Writes the octet's bytes of its block at ob[ib]: the width byte (octet 0),
+// its sign byte, its whole bytes of mag >> r (element-major little-endian)
+// and its r-bit residuals (MSB-first), SZp/src/szp_TypeManager.cc:13-94,820-844.
+__device__ __forceinline__ void encode_octet(uint8_t *ob, uint32_t ib, const Oct &s, uint32_t o,
+                                             uint32_t b, const uint32_t mg[8], uint32_t sgn,
+                                             int32_t qhead)
+{
+    if (!s.m) return;
+    if (s.head) {
+        lds_st32(ob + ib, (uint32_t)qhead);
+        return;
+    }
+    if (o == 0) ob[ib] = (uint8_t)b;
+    if (!b) return;
+    const uint32_t S8 = (s.L + 7) >> 3;
+    ob[ib + 1 + o] = (uint8_t)sgn;
+    const uint32_t nb = b >> 3, r = b & 7;
+    uint8_t *p1 = ob + ib + 1 + S8 + 8 * o * nb;
+    uint32_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = mg[i] >> r;  // < 2^(8 nb)
+    if (nb) {
+        if (s.m == 8) {
+            if (nb == 1) {
+                lds_st64(p1, v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24),
+                         v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24));
+            } else if (nb == 2) {
+                lds_st128(p1, v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16),
+                          v[6] | (v[7] << 16));
+            } else if (nb == 3) {
+                lds_st128(p1, v[0] | (v[1] << 24), (v[1] >> 8) | (v[2] << 16),
+                          (v[2] >> 16) | (v[3] << 8), v[4] | (v[5] << 24));
+                lds_st64(p1 + 16, (v[5] >> 8) | (v[6] << 16), (v[6] >> 16) | (v[7] << 8));
+            } else {
+                lds_st128(p1, v[0], v[1], v[2], v[3]);
+                lds_st128(p1 + 16, v[4], v[5], v[6], v[7]);
+            }
+        } else {
+            for (uint32_t i = 0; i < s.m; ++i)
+                for (uint32_t k = 0; k < nb; ++k) p1[i * nb + k] = (uint8_t)(v[i] >> (8 * k));
+        }
+    }
+    if (r) {
+        const uint32_t mask = (1u << r) - 1;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc = (acc << r) | (mg[i] & mask);
+        const uint32_t nbytes = s.m == 8 ? r : (s.m * r + 7) >> 3;
+        // Big-endian bytes of the 8r-bit stream, first byte lowest in w.
+        const uint64_t w = __builtin_bswap64(acc << (64 - 8 * r));
+        lds_st_bytes(ob + ib + 1 + S8 + nb * s.L + o * r, w, nbytes);
+    }
+}
+
+// A tile is g.R rounds of K items.  Each thread owns one octet per round and
+// loads it straight from HBM one round ahead of its use.  Rounds are encoded
+// into the LDS buffer ob (kOutCap bytes) as long as they fit; the first round
+// that does not fit and all later ones are encoded again after the tile's
+// offset is known.  One decoupled look-back per tile places the tile; tiles
+// are ordered by blockIdx.x (each XCD dispatches its workgroups in index
+// order, and every wait is bounded).
 __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
                                                   const float *__restrict__ d_eb, float eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
@@ -408,180 +605,123 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
                                                   uint32_t *__restrict__ ctl,
                                                   uint64_t *__restrict__ d_size)
 {
-    constexpr int kQ = 2052;  // one tile: <= 2048 elements + the previous one + alignment
-    __shared__ int32_t qs[kQ + kQ / 8 + 8];
-    __shared__ __align__(16) uint8_t ob[kOutBufBytes];
+    __shared__ __align__(16) uint8_t ob[kOutCap + 64];
     __shared__ uint32_t s_max[kWG];
     __shared__ uint32_t s_off[kWG];
-    __shared__ uint64_t s_tile, s_prefix;
-    __shared__ uint32_t s_total;
+    __shared__ uint32_t s_tot;
+    __shared__ uint32_t s_rbase[kMaxRounds + 1];
+    __shared__ uint32_t s_head_t[kWG], s_head_off[kWG];
+    __shared__ uint32_t s_nheads;
+    __shared__ uint64_t s_prefix;
 
     const int tid = threadIdx.x;
-    if (tid == 0)
-        s_tile = __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t tile = blockIdx.x;
     s_max[tid] = 0;
-    s_off[tid] = 0;
-    __syncthreads();
-    const uint64_t tile = s_tile;
-    if (tile >= g.ntiles) return;
-
+    if (tid == 0) s_nheads = 0;
     const float eb = d_eb ? *d_eb : eb_host;
     const float inv = 1.0f / eb;  // FP32 division, SZp/src/szp_float.cc:185
-    const uint64_t item0 = tile * g.K;
-    const uint64_t itemN = min(item0 + g.K, g.nitems);
-    uint32_t t0;
-    uint64_t j0;
-    locate(g, item0, t0, j0);
-    const uint64_t E0 = item_elem(g, t0, j0);
-    const uint64_t E1 = item_start(g, itemN);
-    const uint64_t base = E0 ? E0 - 1 : 0;
-
-    // Stage q = quantize(x) for elements [base, E1) in LDS (padded index).
-    for (uint64_t v = (base & ~3ull) + 4ull * tid; v < E1; v += 4ull * kWG) {
-        float f[4];
-        if (VEC && v + 4 <= g.n) {
-            float4 q = *reinterpret_cast<const float4 *>(x + v);
-            f[0] = q.x; f[1] = q.y; f[2] = q.z; f[3] = q.w;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) f[k] = (v + k < g.n) ? x[v + k] : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t e = v + k;
-            if (e >= base && e < E1) {
-                const uint32_t r = (uint32_t)(e - base);
-                qs[r + (r >> 3)] = quantize(f[k], inv);
-            }
-        }
-    }
-
-    // Slot -> (item, octet).
+    const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
+    const uint64_t tIN = min(tI0 + (uint64_t)g.K * g.R, g.nitems);
     const uint32_t d = ((uint32_t)tid * g.spb_magic) >> 16;
     const uint32_t o = (uint32_t)tid - d * g.spb;
-    const bool active = d < g.K && item0 + d < itemN;
-    uint32_t t = t0;
-    uint64_t j = j0;
-    if (active) advance(g, t, j, d);
-    const bool head = active && j == 0;
-    const uint32_t L = active ? item_len(g, t, j) : 0;
-    const uint64_t es = active ? item_elem(g, t, j) : 0;
+    uint32_t ct;
+    uint64_t cj;
+    locate(g, tI0, ct, cj);
+    uint32_t nit = (uint32_t)min<uint64_t>(g.K, tIN - tI0);
+    Oct cur = octet_of(g, ct, cj, nit, d, o);
+    OctIn cin;
+    octet_load(x, cur, cin);
     __syncthreads();
 
-    // Lorenzo deltas, signs, magnitudes and the block maximum.
-    uint32_t mg[8];
-    uint32_t sgn = 0, m = 0;
-    int32_t qhead = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) mg[i] = 0;
-    if (active && !head && 8 * o < L) {
-        m = min(8u, L - 8 * o);
-        const uint32_t r0 = (uint32_t)(es + 8 * o - base);
-        int32_t prev = qs[(r0 - 1) + ((r0 - 1) >> 3)];
-        uint32_t omax = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if ((uint32_t)i < m) {
-                const uint32_t r = r0 + i;
-                const int32_t c = qs[r + (r >> 3)];
-                const uint32_t dd = (uint32_t)c - (uint32_t)prev;
-                prev = c;
-                const bool neg = (int32_t)dd < 0;
-                mg[i] = neg ? 0u - dd : dd;
-                sgn |= (uint32_t)neg << (7 - i);
-                omax = max(omax, mg[i]);
+    uint32_t run = 0;   // tile-relative byte offset of the current round
+    uint32_t nbuf = 0;  // rounds held in ob
+    bool deferring = false;
+    for (uint32_t r = 0; r < g.R; ++r) {
+        const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
+        if (ri0 >= tIN) break;
+        nit = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
+        // Next round's octet: geometry and loads issued before this round's work.
+        uint32_t nt = ct;
+        uint64_t nj = cj;
+        advance(g, nt, nj, nit);
+        const uint64_t rn0 = ri0 + nit;
+        Oct nxt{0, 0, 0, 0, false, false};
+        OctIn nin;
+        if (rn0 < tIN) {
+            nxt = octet_of(g, nt, nj, (uint32_t)min<uint64_t>(g.K, tIN - rn0), d, o);
+            octet_load(x, nxt, nin);
+        }
+
+        uint32_t mg[8], sgn;
+        int32_t qh;
+        s_off[tid] = 0;  // the scan reads all kWG entries; items >= K contribute 0
+        const uint32_t omax = octet_deltas(cur, cin, inv, mg, sgn, qh);
+        if (omax) atomicMax(&s_max[d], omax);
+        __syncthreads();
+        uint32_t b = 0, len = 0;
+        if (cur.live) {
+            if (cur.head) {
+                len = 4;
+            } else {
+                b = bit_width(s_max[d]);
+                len = block_bytes(b, cur.L);
             }
         }
-        if (omax) atomicMax(&s_max[d], omax);
-    }
-    if (head && o == 0) {
-        const uint32_t r = (uint32_t)(es - base);
-        qhead = qs[r + (r >> 3)];
-    }
-    __syncthreads();
-
-    uint32_t b = 0;
-    if (active) {
-        uint32_t len = 4;
-        if (!head) {
-            b = bit_width(s_max[d]);
-            len = block_bytes(b, L);
+        if (o == 0 && d < g.K) s_off[d] = len;
+        __syncthreads();
+        s_max[tid] = 0;
+        if (tid < 64) {
+            const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
+            const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
+            const uint32_t sum = a0 + a1 + a2 + a3;
+            const uint32_t incl = wave_incl_scan_u32(sum, tid);
+            const uint32_t ex = incl - sum;
+            s_off[4 * tid] = ex;
+            s_off[4 * tid + 1] = ex + a0;
+            s_off[4 * tid + 2] = ex + a0 + a1;
+            s_off[4 * tid + 3] = ex + a0 + a1 + a2;
+            if (tid == 63) s_tot = incl;
         }
-        if (o == 0) s_off[d] = len;
+        __syncthreads();
+        const uint32_t rt = s_tot;
+        const uint32_t ib = cur.live ? s_off[d] : 0;
+        if (cur.head && o == 0) {
+            const uint32_t k = atomicAdd(&s_nheads, 1u);
+            s_head_t[k] = cur.t;
+            s_head_off[k] = run + ib;
+        }
+        if (tid == 0) s_rbase[r] = run;
+        if (!deferring && run + rt <= (uint32_t)kOutCap) {
+            encode_octet(ob, run + ib, cur, o, b, mg, sgn, qh);
+            nbuf = r + 1;
+        } else {
+            deferring = true;
+        }
+        run += rt;
+        cur = nxt;
+        cin = nin;
+        ct = nt;
+        cj = nj;
+        __syncthreads();
     }
-    __syncthreads();
 
-    // Exclusive scan of item byte lengths inside the tile.
     if (tid < 64) {
-        const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
-        const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
-        const uint32_t sum = a0 + a1 + a2 + a3;
-        const uint32_t incl = wave_incl_scan_u32(sum, tid);
-        const uint32_t ex = incl - sum;
-        s_off[4 * tid] = ex;
-        s_off[4 * tid + 1] = ex + a0;
-        s_off[4 * tid + 2] = ex + a0 + a1;
-        s_off[4 * tid + 3] = ex + a0 + a1 + a2;
-        if (tid == 63) s_total = incl;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        const uint64_t p = lookback_sum(status, tile, 0, s_total, ctl);
+        const uint64_t p = lookback_sum(status, tile, 0, run, ctl);
         if (tid == 0) s_prefix = p;
     }
     __syncthreads();
-    const uint64_t P = s_prefix;
-    const uint32_t tot = s_total;
-    const uint64_t G0 = g.body + P;
-    if (itemN == g.nitems && tid == 0) *d_size = G0 + tot;
-    if (G0 + tot > cap || (ld_ctl(&ctl[1]) & kErrTimeout)) {
-        if (tid == 0 && G0 + tot > cap) raise_error(ctl, kErrCapacity);
+    const uint64_t G0 = g.body + s_prefix;
+    if (tIN == g.nitems && tid == 0) *d_size = G0 + run;
+    if (G0 + run > cap || (ld_ctl(&ctl[1]) & kErrTimeout)) {
+        if (tid == 0 && G0 + run > cap) raise_error(ctl, kErrCapacity);
         return;
     }
-
-    // Encode into LDS; every output byte of the tile is written exactly once.
-    if (active) {
-        const uint32_t ib = s_off[d];
-        if (head) {
-            if (o == 0) {
-                ob[ib] = (uint8_t)qhead;
-                ob[ib + 1] = (uint8_t)((uint32_t)qhead >> 8);
-                ob[ib + 2] = (uint8_t)((uint32_t)qhead >> 16);
-                ob[ib + 3] = (uint8_t)((uint32_t)qhead >> 24);
-                // Chunk offset relative to the body (SZp/src/szp_float.cc:280-288).
-                const uint64_t off = P + ib;
-                uint32_t *h = reinterpret_cast<uint32_t *>(out + 4 + 8ull * t);
-                h[0] = (uint32_t)off;
-                h[1] = (uint32_t)(off >> 32);
-            }
-        } else {
-            if (o == 0) ob[ib] = (uint8_t)b;
-            if (b && m) {
-                const uint32_t S8 = (L + 7) >> 3;
-                ob[ib + 1 + o] = (uint8_t)sgn;
-                const uint32_t nb = b >> 3, r = b & 7;
-                uint32_t p1 = ib + 1 + S8 + 8 * o * nb;
-                if (nb) {
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        if ((uint32_t)i < m) {
-                            const uint32_t v = mg[i] >> r;
-                            for (uint32_t k = 0; k < nb; ++k) ob[p1++] = (uint8_t)(v >> (8 * k));
-                        }
-                    }
-                }
-                if (r) {
-                    const uint32_t mask = (1u << r) - 1;
-                    uint64_t acc = 0;
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) acc = (acc << r) | (mg[i] & mask);
-                    const uint32_t nbytes = (m * r + 7) >> 3;
-                    const uint32_t p2 = ib + 1 + S8 + nb * L + o * r;
-                    for (uint32_t k = 0; k < nbytes; ++k)
-                        ob[p2 + k] = (uint8_t)(acc >> (8 * (r - 1 - k)));
-                }
-            }
-        }
+    // Chunk offsets relative to the body (SZp/src/szp_float.cc:280-288).
+    for (uint32_t k = tid; k < s_nheads; k += kWG) {
+        const uint64_t off = s_prefix + s_head_off[k];
+        uint32_t *h = reinterpret_cast<uint32_t *>(out + 4 + 8ull * s_head_t[k]);
+        h[0] = (uint32_t)off;
+        h[1] = (uint32_t)(off >> 32);
     }
     if (tile == 0) {
         if (tid == 0) {
@@ -600,8 +740,56 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
             }
         }
     }
-    __syncthreads();
-    copy_out_bytes(out, G0, tot, ob, tid);
+    const bool all_buffered = tI0 + (uint64_t)nbuf * g.K >= tIN;
+    const uint32_t buffered = all_buffered ? run : s_rbase[nbuf];
+    copy_out_bytes(out, G0, buffered, ob, tid);
+    // Rounds that did not fit: encode again and write them one at a time.
+    for (uint32_t r = nbuf; r < g.R; ++r) {
+        const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
+        if (ri0 >= tIN) break;
+        nit = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
+        uint32_t rt0;
+        uint64_t rj0;
+        locate(g, ri0, rt0, rj0);
+        const Oct s = octet_of(g, rt0, rj0, nit, d, o);
+        OctIn in;
+        octet_load(x, s, in);
+        uint32_t mg[8], sgn;
+        int32_t qh;
+        __syncthreads();
+        s_off[tid] = 0;
+        const uint32_t omax = octet_deltas(s, in, inv, mg, sgn, qh);
+        if (omax) atomicMax(&s_max[d], omax);
+        __syncthreads();
+        uint32_t b = 0, len = 0;
+        if (s.live) {
+            if (s.head) {
+                len = 4;
+            } else {
+                b = bit_width(s_max[d]);
+                len = block_bytes(b, s.L);
+            }
+        }
+        if (o == 0 && d < g.K) s_off[d] = len;
+        __syncthreads();
+        s_max[tid] = 0;
+        if (tid < 64) {
+            const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
+            const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
+            const uint32_t sum = a0 + a1 + a2 + a3;
+            const uint32_t incl = wave_incl_scan_u32(sum, tid);
+            const uint32_t ex = incl - sum;
+            s_off[4 * tid] = ex;
+            s_off[4 * tid + 1] = ex + a0;
+            s_off[4 * tid + 2] = ex + a0 + a1;
+            s_off[4 * tid + 3] = ex + a0 + a1 + a2;
+            if (tid == 63) s_tot = incl;
+        }
+        __syncthreads();
+        encode_octet(ob, s.live ? s_off[d] : 0, s, o, b, mg, sgn, qh);
+        __syncthreads();
+        copy_out_bytes(out, G0 + s_rbase[r], s_tot, ob, tid);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -687,8 +875,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     __shared__ int s_spec_ok, s_nonsync, s_broken, s_merged;
 
     const int tid = threadIdx.x;
-    if (tid == 0)
-        s_tile = __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) s_tile = blockIdx.x;
     if (tid < 33) s_lenB[tid] = block_bytes(tid, g.B);
     if (tid == 0) { s_nonsync = 0; s_spec_ok = 0; s_spec_cnt = 0; s_spec_exit = 0; }
     __syncthreads();
@@ -863,154 +1050,95 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     }
 }
 
-template <bool VEC>
-__global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
-                                                Geom g, const uint64_t *__restrict__ ctab,
-                                                const uint64_t *__restrict__ blkpos,
-                                                float *__restrict__ out,
-                                                uint64_t *__restrict__ status,
-                                                uint32_t *__restrict__ ctl)
+// Unpacks one slot (SZp/src/szpd_float.cc:81-110, SZp/src/szp_TypeManager.cc:436-944).
+// The slot's item starts at buf[lp]; lend bounds the bytes present (reads may
+// run up to 32 bytes past it, inside the buffer's slack).  dl[i] = inclusive
+// prefix of the slot's deltas, or the absolute head value in dl[0] for a head
+// slot (h = 1).  Returns the number of values (0 for idle slots); sets bad on
+// an invalid width or a block that overruns the bytes present.
+__device__ __forceinline__ uint32_t decode_slot(uint32_t L, bool head, uint32_t o,
+                                                const uint8_t *buf, uint32_t lp, uint32_t lend,
+                                                uint32_t dl[8], uint32_t &h, bool &bad)
 {
-    __shared__ __align__(16) uint8_t buf[kOutBufBytes + 32];
-    __shared__ __align__(16) float fs[2048 + 8];
-    __shared__ uint64_t s_ipos[kWG];
-    __shared__ uint32_t s_wh[kWG / 64], s_wv[kWG / 64];
-    __shared__ uint64_t s_tile, s_P0, s_P1;
-    __shared__ uint32_t s_carry, s_bad;
-
-    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
-    if (tid == 0) {
-        s_tile = __hip_atomic_fetch_add(&ctl[3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_bad = 0;
-    }
-    __syncthreads();
-    const uint64_t tile = s_tile;
-    if (tile >= g.ntiles) return;
-    if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
-        // Successors may already wait on this tile: publish before leaving.
-        if (tid == 0) st_status(&status[tile], kFlagIncl);
-        return;
-    }
-
-    const uint32_t ebu = ((uint32_t)st[0] << 24) | ((uint32_t)st[1] << 16) |
-                         ((uint32_t)st[2] << 8) | (uint32_t)st[3];
-    const float eb = __uint_as_float(ebu);
-    const uint64_t item0 = tile * g.K;
-    const uint64_t itemN = min(item0 + g.K, g.nitems);
-    const uint32_t nit = (uint32_t)(itemN - item0);
-    uint32_t t0;
-    uint64_t j0;
-    locate(g, item0, t0, j0);
-    const uint64_t E0 = item_elem(g, t0, j0);
-    const uint64_t E1 = item_start(g, itemN);
-    const uint64_t Eb = E0 & ~3ull;
-
-    const uint32_t d = ((uint32_t)tid * g.spb_magic) >> 16;
-    const uint32_t o = (uint32_t)tid - d * g.spb;
-    const bool active = d < g.K && d < nit;
-    uint32_t t = t0;
-    uint64_t j = j0;
-    if (active) advance(g, t, j, d);
-    const bool head = active && j == 0;
-    const uint32_t L = active ? item_len(g, t, j) : 0;
-    const uint64_t es = active ? item_elem(g, t, j) : 0;
-    if (active && o == 0) s_ipos[d] = head ? ctab[t] : blkpos[(uint64_t)t * g.nb0 + (j - 1)];
-    __syncthreads();
-    if (tid == 0) {
-        const uint64_t P0 = s_ipos[0], Pl = s_ipos[nit - 1];
-        uint32_t tl;
-        uint64_t jl;
-        locate(g, itemN - 1, tl, jl);
-        uint64_t P1;
-        if (jl == 0) {
-            P1 = Pl + 4;
-        } else {
-            const uint32_t b = Pl < len ? st[Pl] : 255u;
-            P1 = b > 32 ? ~0ull : Pl + block_bytes(b, item_len(g, tl, jl));
-        }
-        if (P1 > len || P0 > Pl || P1 - P0 > kOutBufBytes) {
-            s_bad = 1;
-            raise_error(ctl, kErrStream);
-        }
-        s_P0 = P0;
-        s_P1 = P1;
-    }
-    __syncthreads();
-    // A malformed stream stops here; later tiles may wait on this one, so
-    // publish an inclusive zero first.
-    if (s_bad) {
-        if (tid == 0) st_status(&status[tile], kFlagIncl);
-        return;
-    }
-    const uint64_t P0 = s_P0, P1 = s_P1;
-    const uint64_t pb = P0 & ~15ull;
-    for (uint64_t a = pb + 16ull * tid; a < P1; a += 16ull * kWG) {
-        if (a + 16 <= len) {
-            *reinterpret_cast<uint4 *>(buf + (a - pb)) = *reinterpret_cast<const uint4 *>(st + a);
-        } else {
-            for (int k = 0; k < 16; ++k)
-                if (a + k < len) buf[a - pb + k] = st[a + k];
-        }
-    }
-    __syncthreads();
-
-    // Unpack this slot's (up to) 8 deltas (SZp/src/szpd_float.cc:81-110).
-    int32_t dl[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) dl[i] = 0;
-    uint32_t m = 0;
-    uint32_t hflag = 0, agg = 0;
-    if (active) {
-        const uint32_t lp = (uint32_t)(s_ipos[d] - pb);
-        if (head) {
-            if (o == 0) {
-                m = 1;
-                hflag = 1;
-                agg = (uint32_t)buf[lp] | ((uint32_t)buf[lp + 1] << 8) |
-                      ((uint32_t)buf[lp + 2] << 16) | ((uint32_t)buf[lp + 3] << 24);
-            }
-        } else if (8 * o < L) {
-            m = min(8u, L - 8 * o);
-            const uint32_t b = buf[lp];
-            if (b > 32 || s_ipos[d] + block_bytes(b, L) > P1) {
-                s_bad = 1;
-            } else if (b) {
-                const uint32_t S8 = (L + 7) >> 3;
-                const uint32_t sg = buf[lp + 1 + o];
-                const uint32_t nb = b >> 3, r = b & 7;
-                const uint32_t p1 = lp + 1 + S8 + 8 * o * nb;
-                uint64_t acc = 0;
-                if (r) {
-                    const uint32_t nbytes = (m * r + 7) >> 3;
-                    const uint32_t p2 = lp + 1 + S8 + nb * L + o * r;
-                    for (uint32_t k = 0; k < nbytes; ++k) acc = (acc << 8) | buf[p2 + k];
-                    acc <<= 8 * (r - nbytes);
-                }
-                const uint32_t mask = (1u << r) - 1;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    if ((uint32_t)i < m) {
-                        uint32_t v = 0;
-                        for (uint32_t k = 0; k < nb; ++k) v |= (uint32_t)buf[p1 + i * nb + k] << (8 * k);
-                        const uint32_t low = r ? (uint32_t)(acc >> (r * (7 - i))) & mask : 0u;
-                        const uint32_t mag = (r ? (v << r) : v) | low;
-                        const bool neg = (sg >> (7 - i)) & 1u;
-                        dl[i] = (int32_t)(neg ? 0u - mag : mag);
-                    }
-                }
-            }
-        }
+    h = 0;
+    if (head) {
+        if (o) return 0;
+        h = 1;
+        __builtin_memcpy(&dl[0], buf + lp, 4);
+        return 1;
     }
-    // Inclusive prefix inside the slot, then a segmented scan over slots:
-    // a head slot restarts the running value with its absolute q.
+    if (8 * o >= L) return 0;
+    const uint32_t m = min(8u, L - 8 * o);
+    const uint32_t b = buf[lp];
+    if (b > 32 || lp + block_bytes(b, L) > lend) {
+        bad = true;
+        return m;
+    }
+    if (!b) return m;
+    const uint32_t S8 = (L + 7) >> 3;
+    const uint32_t sg = buf[lp + 1 + o];
+    const uint32_t nb = b >> 3, r = b & 7;
+    const uint8_t *p1 = buf + lp + 1 + S8 + 8 * o * nb;
+    uint32_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = 0;
+    if (nb == 1) {
+        const uint2 w = lds_ld64(p1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            v[i] = (w.x >> (8 * i)) & 0xFFu;
+            v[i + 4] = (w.y >> (8 * i)) & 0xFFu;
+        }
+    } else if (nb == 2) {
+        const uint4 w = lds_ld128(p1);
+        v[0] = w.x & 0xFFFFu; v[1] = w.x >> 16;
+        v[2] = w.y & 0xFFFFu; v[3] = w.y >> 16;
+        v[4] = w.z & 0xFFFFu; v[5] = w.z >> 16;
+        v[6] = w.w & 0xFFFFu; v[7] = w.w >> 16;
+    } else if (nb == 3) {
+        const uint4 a = lds_ld128(p1);
+        const uint2 c = lds_ld64(p1 + 16);
+        v[0] = a.x & 0xFFFFFFu;
+        v[1] = __builtin_amdgcn_alignbyte(a.y, a.x, 3) & 0xFFFFFFu;
+        v[2] = __builtin_amdgcn_alignbyte(a.z, a.y, 2) & 0xFFFFFFu;
+        v[3] = a.z >> 8;
+        v[4] = a.w & 0xFFFFFFu;
+        v[5] = __builtin_amdgcn_alignbyte(c.x, a.w, 3) & 0xFFFFFFu;
+        v[6] = __builtin_amdgcn_alignbyte(c.y, c.x, 2) & 0xFFFFFFu;
+        v[7] = c.y >> 8;
+    } else if (nb == 4) {
+        const uint4 a = lds_ld128(p1), c = lds_ld128(p1 + 16);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
+    }
+    uint64_t A = 0;
+    if (r) {
+        const uint2 w = lds_ld64(buf + lp + 1 + S8 + nb * L + o * r);
+        A = __builtin_bswap64(((uint64_t)w.y << 32) | w.x);  // stream bits MSB-first
+    }
+    const uint32_t mask = (1u << r) - 1;
     uint32_t run = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        run += (uint32_t)dl[i];
-        dl[i] = (int32_t)run;
+        const uint32_t low = r ? (uint32_t)(A >> (64 - r * (i + 1))) & mask : 0u;
+        const uint32_t mag = (v[i] << r) | low;
+        const bool neg = (sg >> (7 - i)) & 1u;
+        if ((uint32_t)i < m) run += neg ? 0u - mag : mag;
+        dl[i] = run;
     }
-    if (!hflag) agg = run;
-    uint32_t h = hflag, v = agg;
+    return m;
+}
+
+// Segmented scan of slot aggregates (h, v) in slot order: a head restarts
+// the running value.  Returns the slot's exclusive (eh, ev) and the round's
+// total (rh, rv).
+__device__ __forceinline__ void seg_scan_round(uint32_t h, uint32_t v, uint32_t *s_wh,
+                                               uint32_t *s_wv, uint32_t &eh, uint32_t &ev,
+                                               uint32_t &rh, uint32_t &rv)
+{
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
 #pragma unroll
     for (int dd = 1; dd < 64; dd <<= 1) {
         const uint32_t ph = __shfl_up(h, dd), pv = __shfl_up(v, dd);
@@ -1019,80 +1147,268 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
             h |= ph;
         }
     }
-    if (lane == 63) { s_wh[wv] = h; s_wv[wv] = v; }
+    if (lane == 63) {
+        s_wh[wv] = h;
+        s_wv[wv] = v;
+    }
     __syncthreads();
-    // Exclusive (h, v) of this slot within the tile.
-    uint32_t eh = __shfl_up(h, 1), ev = __shfl_up(v, 1);
-    if (lane == 0) { eh = 0; ev = 0; }
-    {
-        uint32_t wh = 0, wvv = 0;
-        for (int k = 0; k < wv; ++k) {
-            wvv = s_wh[k] ? s_wv[k] : wvv + s_wv[k];
-            wh |= s_wh[k];
-        }
-        ev = eh ? ev : wvv + ev;
-        eh |= wh;
+    eh = __shfl_up(h, 1);
+    ev = __shfl_up(v, 1);
+    if (lane == 0) {
+        eh = 0;
+        ev = 0;
     }
-    uint32_t th = 0, tv = 0;
+    uint32_t wh = 0, wvv = 0;
+    for (int k = 0; k < wv; ++k) {
+        wvv = s_wh[k] ? s_wv[k] : wvv + s_wv[k];
+        wh |= s_wh[k];
+    }
+    ev = eh ? ev : wvv + ev;
+    eh |= wh;
+    rh = 0;
+    rv = 0;
     for (int k = 0; k < kWG / 64; ++k) {
-        tv = s_wh[k] ? s_wv[k] : tv + s_wv[k];
-        th |= s_wh[k];
+        rv = s_wh[k] ? s_wv[k] : rv + s_wv[k];
+        rh |= s_wh[k];
     }
-    // Look-back: a tile whose aggregate contains a head is absolute by itself.
-    if (wv == 0) {
-        if (th && lane == 0) st_status(&status[tile], kFlagIncl | tv);
+    __syncthreads();
+}
+
+// Copies stream bytes [a, b) to buf[0, b - a) (16-byte loads, 16-byte aligned
+// LDS destination offset base_sh = a & 15 so buf[p - (a & ~15)] holds p).
+__device__ __forceinline__ void load_bytes(const uint8_t *__restrict__ st, uint64_t len,
+                                           uint64_t a, uint64_t b, uint8_t *buf)
+{
+    const uint64_t a0 = a & ~15ull;
+    for (uint64_t q = a0 + 16ull * threadIdx.x; q < b; q += 16ull * kWG) {
+        if (q + 16 <= len) {
+            *reinterpret_cast<uint4 *>(buf + (q - a0)) = *reinterpret_cast<const uint4 *>(st + q);
+        } else {
+            for (int k = 0; k < 16; ++k)
+                if (q + k < len) buf[q - a0 + k] = st[q + k];
+        }
+    }
+}
+
+// Decodes tiles of g.R rounds of K items.  The item positions of the tile
+// (chunk heads from the header, blocks from the parse) and, when they fit,
+// all its stream bytes are loaded once.  Pass 1 unpacks every round to get
+// its segmented delta sum; one look-back gives the tile's carry (the q of the
+// element before the tile); pass 2 unpacks again and writes
+// fl32((float)q * eb).
+template <bool VEC>
+__global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
+                                                Geom g, const uint64_t *__restrict__ ctab,
+                                                const uint64_t *__restrict__ blkpos,
+                                                float *__restrict__ out,
+                                                uint64_t *__restrict__ status,
+                                                uint32_t *__restrict__ ctl)
+{
+    __shared__ __align__(16) uint8_t buf[kDecTileBytes + 64];
+    __shared__ __align__(16) float fs[2048 + 8];
+    __shared__ uint32_t s_pos[kWG * kMaxRounds];  // item positions - tile base
+    __shared__ uint32_t s_wh[kWG / 64], s_wv[kWG / 64];
+    __shared__ uint32_t s_rxh[kMaxRounds], s_rxv[kMaxRounds];
+    __shared__ uint64_t s_base, s_end;
+    __shared__ uint64_t s_rE[kMaxRounds + 1];  // first element of each round
+    __shared__ uint32_t s_carry;
+    __shared__ int s_bad;
+
+    const int tid = threadIdx.x, lane = __lane_id();
+    const uint64_t tile = blockIdx.x;
+    if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
+        // Successors may wait on this tile: publish before leaving.
+        if (tid == 0) st_status(&status[tile], kFlagIncl);
+        return;
+    }
+    const uint32_t ebu = ((uint32_t)st[0] << 24) | ((uint32_t)st[1] << 16) |
+                         ((uint32_t)st[2] << 8) | (uint32_t)st[3];
+    const float eb = __uint_as_float(ebu);
+    const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
+    const uint64_t tIN = min(tI0 + (uint64_t)g.K * g.R, g.nitems);
+    const uint32_t nti = (uint32_t)(tIN - tI0);
+    uint32_t t0;
+    uint64_t j0;
+    locate(g, tI0, t0, j0);
+    if (tid <= (int)kMaxRounds) {
+        const uint64_t it = min(tI0 + (uint64_t)tid * g.K, tIN);
+        uint32_t t = t0;
+        uint64_t j = j0;
+        advance(g, t, j, (uint32_t)(it - tI0));
+        s_rE[tid] = it >= g.nitems ? g.n : item_elem(g, t, j);
+    }
+
+    // Item positions of the whole tile.
+    uint64_t mypos[kMaxRounds];
+#pragma unroll
+    for (int k = 0; k < kMaxRounds; ++k) {
+        const uint32_t it = tid + kWG * k;
+        mypos[k] = 0;
+        if (it < nti) {
+            uint32_t t = t0;
+            uint64_t j = j0;
+            advance(g, t, j, it);
+            mypos[k] = j == 0 ? ctab[t] : blkpos[(uint64_t)t * g.nb0 + (j - 1)];
+            if (it == 0) s_base = mypos[k];
+            if (it == nti - 1) {
+                uint64_t e;
+                if (j == 0) {
+                    e = mypos[k] + 4;
+                } else {
+                    const uint32_t b = mypos[k] < len ? st[mypos[k]] : 255u;
+                    e = b > 32 ? ~0ull : mypos[k] + block_bytes(b, item_len(g, t, j));
+                }
+                s_end = e;
+            }
+        }
+    }
+    __syncthreads();
+    const uint64_t tbase = s_base & ~15ull, tend = s_end;
+    const bool whole = tend <= len && s_base <= tend && tend - tbase <= (uint64_t)kDecTileBytes;
+    bool bad = !(tend <= len && s_base <= tend);
+#pragma unroll
+    for (int k = 0; k < kMaxRounds; ++k) {
+        const uint32_t it = tid + kWG * k;
+        if (it < nti) {
+            if (mypos[k] < tbase || mypos[k] - tbase >= (1ull << 32)) bad = true;
+            s_pos[it] = (uint32_t)(mypos[k] - tbase);
+        }
+    }
+    if (whole) load_bytes(st, len, s_base, tend, buf);
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    if (bad) s_bad = 1;
+    __syncthreads();
+
+    // Slot -> item within a round.
+    const uint32_t d = ((uint32_t)tid * g.spb_magic) >> 16;
+    const uint32_t o = (uint32_t)tid - d * g.spb;
+
+    for (int pass = 0; pass < 2 && !s_bad; ++pass) {
+        uint32_t th = 0, tv = 0;
         uint32_t carry = 0;
-        if (tile > 0 && !(j0 == 0)) {
-            if (!th && lane == 0) st_status(&status[tile], kFlagAgg | tv);
-            int64_t hi = (int64_t)tile - 1;
-            while (true) {
-                const int64_t idx = hi - lane;
-                const uint64_t sx = idx >= 0 ? wait_status(&status[idx], ctl) : kFlagIncl;
-                const uint64_t stop = __ballot((sx >> 62) == 2 || (sx & kHeadBit));
-                if (stop) {
-                    const int f = __ffsll((long long)stop) - 1;
-                    carry += wave_sum_u32(lane <= f ? (uint32_t)sx : 0u);
+        if (pass == 1) carry = s_carry;
+        for (uint32_t r = 0; r < g.R; ++r) {
+            const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
+            if (ri0 >= tIN) break;
+            const uint64_t riN = min(ri0 + g.K, tIN);
+            const uint32_t rfirst = r * g.K, rn = (uint32_t)(riN - ri0);
+            // Bytes of this round: all of the tile, or this round alone.
+            uint32_t rb = 0, rend = (uint32_t)(tend - tbase);
+            if (!whole) {
+                const uint64_t a = tbase + s_pos[rfirst];
+                const uint64_t b = riN < tIN ? tbase + s_pos[rfirst + rn] : tend;
+                if (b - (a & ~15ull) > (uint64_t)kDecTileBytes) {
+                    if (tid == 0) s_bad = 1;
                     break;
                 }
-                carry += wave_sum_u32((uint32_t)sx);
-                hi -= 64;
+                __syncthreads();
+                load_bytes(st, len, a, b, buf);
+                __syncthreads();
+                rb = (uint32_t)((a & ~15ull) - tbase);
+                rend = (uint32_t)(b - tbase);
             }
-            if (!th && lane == 0) st_status(&status[tile], kFlagIncl | (uint32_t)(carry + tv));
-        } else if (!th && lane == 0) {
-            st_status(&status[tile], kFlagIncl | tv);
-        }
-        if (lane == 0) s_carry = carry;
-    }
-    // (th is always set for the tile holding item 0; the AGG flag of a tile
-    // whose aggregate contains a head is never needed since it publishes INCL.)
-    __syncthreads();
-    const uint32_t carry = s_carry;
-    if (m) {
-        const uint32_t before = eh ? ev : carry + ev;
-        const uint32_t base_e = (uint32_t)(es + (head ? 0 : 8 * o) - Eb);
-        if (head) {
-            fs[base_e] = (float)(int32_t)agg * eb;
-        } else {
+            const bool active = d < g.K && d < rn;
+            uint32_t t = t0, L = 0;
+            uint64_t j = j0, es = 0;
+            if (active) {
+                advance(g, t, j, rfirst + d);
+                L = item_len(g, t, j);
+                es = item_elem(g, t, j);
+            }
+            uint32_t dl[8], h = 0, m = 0;
+            bool b2 = false;
+            (void)0;
+            if (active) m = decode_slot(L, j == 0, o, buf, s_pos[rfirst + d] - rb, rend - rb, dl, h, b2);
+            if (b2) s_bad = 1;
+            const uint32_t v = m ? (h ? dl[0] : dl[m - 1]) : 0u;
+            uint32_t eh, ev, rh, rv;
+            seg_scan_round(h, v, s_wh, s_wv, eh, ev, rh, rv);
+            if (pass == 0) {
+                if (tid == 0) {
+                    s_rxh[r] = th;
+                    s_rxv[r] = tv;
+                }
+            } else {
+                // carry (+) tile prefix of the round (+) slot prefix in the round
+                uint32_t before = s_rxh[r] ? s_rxv[r] : carry + s_rxv[r];
+                before = eh ? ev : before + ev;
+                const uint64_t E0 = s_rE[r], E1 = s_rE[r + 1];
+                const uint64_t Eb = E0 & ~3ull;
+                if (m) {
+                    const uint32_t fo = (uint32_t)(es - Eb) + (h ? 0u : 8 * o);
+                    if (h) {
+                        fs[fo] = (float)(int32_t)dl[0] * eb;
+                    } else if (m == 8) {
+                        float4 a, c;
+                        a.x = (float)(int32_t)(before + dl[0]) * eb;
+                        a.y = (float)(int32_t)(before + dl[1]) * eb;
+                        a.z = (float)(int32_t)(before + dl[2]) * eb;
+                        a.w = (float)(int32_t)(before + dl[3]) * eb;
+                        c.x = (float)(int32_t)(before + dl[4]) * eb;
+                        c.y = (float)(int32_t)(before + dl[5]) * eb;
+                        c.z = (float)(int32_t)(before + dl[6]) * eb;
+                        c.w = (float)(int32_t)(before + dl[7]) * eb;
+                        __builtin_memcpy(&fs[fo], &a, 16);
+                        __builtin_memcpy(&fs[fo + 4], &c, 16);
+                    } else {
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if ((uint32_t)i < m) fs[base_e + i] = (float)(int32_t)(before + (uint32_t)dl[i]) * eb;
+                        for (int i = 0; i < 8; ++i)
+                            if ((uint32_t)i < m) fs[fo + i] = (float)(int32_t)(before + dl[i]) * eb;
+                    }
+                }
+                __syncthreads();
+                const uint64_t a0 = (E0 + 3) & ~3ull, a1 = E1 & ~3ull;
+                if (!VEC || a0 >= a1) {
+                    for (uint64_t e = E0 + tid; e < E1; e += kWG) out[e] = fs[e - Eb];
+                } else {
+                    if (E0 + tid < a0) out[E0 + tid] = fs[E0 + tid - Eb];
+                    if (a1 + tid < E1) out[a1 + tid] = fs[a1 + tid - Eb];
+                    for (uint64_t e = a0 + 4ull * tid; e < a1; e += 4ull * kWG)
+                        *reinterpret_cast<float4 *>(out + e) =
+                            *reinterpret_cast<const float4 *>(&fs[e - Eb]);
+                }
+                __syncthreads();
+            }
+            tv = rh ? rv : tv + rv;
+            th |= rh;
         }
+        if (pass == 1) break;
+        __syncthreads();
+        if (s_bad) break;
+        // Look-back: a tile whose aggregate contains a head is absolute by itself.
+        if (tid < 64) {
+            if (th && lane == 0) st_status(&status[tile], kFlagIncl | tv);
+            uint32_t c = 0;
+            if (tile > 0 && j0 != 0) {
+                if (!th && lane == 0) st_status(&status[tile], kFlagAgg | tv);
+                int64_t hi = (int64_t)tile - 1;
+                while (true) {
+                    const int64_t idx = hi - lane;
+                    const uint64_t sx = idx >= 0 ? wait_status(&status[idx], ctl) : kFlagIncl;
+                    const uint64_t stop = __ballot((sx >> 62) == 2);
+                    if (stop) {
+                        const int f = __ffsll((long long)stop) - 1;
+                        c += wave_sum_u32(lane <= f ? (uint32_t)sx : 0u);
+                        break;
+                    }
+                    c += wave_sum_u32((uint32_t)sx);
+                    hi -= 64;
+                }
+                if (!th && lane == 0) st_status(&status[tile], kFlagIncl | (uint32_t)(c + tv));
+            } else if (!th && lane == 0) {
+                st_status(&status[tile], kFlagIncl | tv);
+            }
+            if (lane == 0) s_carry = c;
+        }
+        __syncthreads();
     }
     __syncthreads();
-    if (s_bad) {
-        if (tid == 0) raise_error(ctl, kErrStream);
-        return;
+    if (s_bad && tid == 0) {
+        // Publish (harmlessly, if already published) so no successor waits.
+        st_status(&status[tile], kFlagIncl);
+        raise_error(ctl, kErrStream);
     }
-    // Store [E0, E1): aligned float4 middle, scalar ends.
-    const uint64_t a0 = (E0 + 3) & ~3ull, a1 = E1 & ~3ull;
-    if (!VEC || a0 >= a1) {
-        for (uint64_t e = E0 + tid; e < E1; e += kWG) out[e] = fs[e - Eb];
-        return;
-    }
-    if (E0 + tid < a0) out[E0 + tid] = fs[E0 + tid - Eb];
-    if (a1 + tid < E1) out[a1 + tid] = fs[a1 + tid - Eb];
-    for (uint64_t e = a0 + 4ull * tid; e < a1; e += 4ull * kWG)
-        *reinterpret_cast<float4 *>(out + e) = *reinterpret_cast<const float4 *>(&fs[e - Eb]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1118,9 +1434,12 @@ bool make_geom(uint64_t n, int B, int T, Geom *g)
     g->spb = (uint32_t)((B + 7) / 8);
     g->K = (uint32_t)kWG / g->spb;
     g->spb_magic = (65536u + g->spb - 1) / g->spb;
-    g->ntiles = (g->nitems + g->K - 1) / g->K;
+    // Tiles of kMaxRounds rounds, unless chunks are so short that a tile would
+    // hold more chunk heads than one workgroup records (at most kWG).
+    g->R = (g->c0 == 0 || g->c0 * (kWG - 1) >= (uint64_t)g->K * kMaxRounds) ? kMaxRounds : 1;
+    const uint64_t per_tile = (uint64_t)g->K * g->R;
+    g->ntiles = (g->nitems + per_tile - 1) / per_tile;
     g->body = 4 + 8ull * (uint64_t)T;
-    g->pad = 0;
     return true;
 }
 
@@ -1134,6 +1453,113 @@ uint64_t compress_bound(uint64_t n, int B, int T)
     if (g.tbs) total += (uint64_t)(T - 1) * (4 + g.nb0 * per_block + 4 * (g.tbs - 1));
     if (nl) total += 4 + g.nblast * per_block + 4 * (nl - 1);
     return total;
+}
+
+// Optional per-kernel device timing: when enabled, every launch below is
+// bracketed by HIP events on its own stream and the elapsed times are summed
+// per kernel when read back (szp_amd_profile_*).
+namespace prof {
+
+struct Rec {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+std::mutex mu;
+bool on = false;
+std::vector<Rec> recs;
+std::vector<hipEvent_t> pool;
+std::vector<std::pair<std::string, std::pair<uint64_t, double>>> totals;
+
+hipEvent_t take()
+{
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct Scope {
+    Rec r{nullptr, nullptr, nullptr};
+    hipStream_t s;
+    Scope(const char *name, hipStream_t st) : s(st)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!on) return;
+        r.name = name;
+        r.a = take();
+        r.b = take();
+        (void)hipEventRecord(r.a, s);
+    }
+    ~Scope()
+    {
+        if (!r.name) return;
+        (void)hipEventRecord(r.b, s);
+        std::lock_guard<std::mutex> lk(mu);
+        recs.push_back(r);
+    }
+};
+
+void collect_locked()
+{
+    for (auto &r : recs) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        bool found = false;
+        for (auto &t : totals) {
+            if (t.first == r.name) {
+                t.second.first += 1;
+                t.second.second += ms;
+                found = true;
+            }
+        }
+        if (!found) totals.push_back({std::string(r.name), {1, (double)ms}});
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    recs.clear();
+}
+
+}  // namespace prof
+
+#define SZP_LAUNCH(name, kern, grid, block, lds, stream, ...)                    \
+    do {                                                                       \
+        prof::Scope _scope(name, stream);                                      \
+        hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);       \
+    } while (0)
+
+void profile_enable(int enable)
+{
+    std::lock_guard<std::mutex> lk(prof::mu);
+    prof::collect_locked();
+    prof::totals.clear();
+    prof::on = enable != 0;
+}
+
+int profile_count()
+{
+    std::lock_guard<std::mutex> lk(prof::mu);
+    prof::collect_locked();
+    return (int)prof::totals.size();
+}
+
+int profile_entry(int i, char *name, int cap, uint64_t *launches, double *total_ms)
+{
+    std::lock_guard<std::mutex> lk(prof::mu);
+    prof::collect_locked();
+    if (i < 0 || i >= (int)prof::totals.size()) return -1;
+    const auto &t = prof::totals[i];
+    if (name && cap > 0) {
+        snprintf(name, (size_t)cap, "%s", t.first.c_str());
+    }
+    if (launches) *launches = t.second.first;
+    if (total_ms) *total_ms = t.second.second;
+    return 0;
 }
 
 namespace {
@@ -1246,7 +1672,7 @@ int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, d
 
     const uint64_t nst = g.ntiles ? g.ntiles : 1;
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((nst + kWG - 1) / kWG, 4096);
-    hipLaunchKernelGGL(k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
+    SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
     const float *d_eb = nullptr;
     const float eb_host = (float)abs_eb;  // SZp/src/szp.cc:32
     const bool vec = (reinterpret_cast<uintptr_t>(d_x) & 15) == 0;
@@ -1255,22 +1681,19 @@ int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, d
         seg = (seg + 4095) & ~4095ull;
         const int np = (int)((n + seg - 1) / seg);
         if (vec)
-            hipLaunchKernelGGL(k_range_partial<true>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);
+            SZP_LAUNCH("k_range_partial", k_range_partial<true>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);
         else
-            hipLaunchKernelGGL(k_range_partial<false>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);
-        hipLaunchKernelGGL(k_range_final, dim3(1), dim3(kWG), 0, stream, d_x, w.parts, np,
+            SZP_LAUNCH("k_range_partial", k_range_partial<false>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);
+        SZP_LAUNCH("k_range_final", k_range_final, dim3(1), dim3(kWG), 0, stream, d_x, w.parts, np,
                            (float)rel, w.eb);
         d_eb = w.eb;
     }
     if (n == 0) {
-        hipLaunchKernelGGL(k_empty_stream, dim3(1), dim3(kWG), 0, stream, d_out, d_eb, eb_host,
+        SZP_LAUNCH("k_empty_stream", k_empty_stream, dim3(1), dim3(kWG), 0, stream, d_out, d_eb, eb_host,
                            g.T, d_size);
-    } else if (vec) {
-        hipLaunchKernelGGL(k_compress<true>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_x,
-                           d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size);
     } else {
-        hipLaunchKernelGGL(k_compress<false>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_x,
-                           d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size);
+        SZP_LAUNCH("k_compress", k_compress, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_x,
+                   d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size);
     }
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
@@ -1289,19 +1712,19 @@ int launch_decompress(const uint8_t *d_stream, uint64_t len, uint64_t n, int B, 
     const uint32_t S = parse_tile_bytes(B);
     const uint32_t W = block_bytes(32, (uint32_t)B);
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((w.nclr + kWG - 1) / kWG, 4096);
-    hipLaunchKernelGGL(k_dsetup, dim3(cgrid ? cgrid : 1), dim3(kWG), 0, stream, d_stream, len, g,
+    SZP_LAUNCH("k_dsetup", k_dsetup, dim3(cgrid ? cgrid : 1), dim3(kWG), 0, stream, d_stream, len, g,
                        S, w.ctab, w.exit_st, w.nclr, w.ctl);
     if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;
     const uint64_t nt1 = len / S + 1 + g.T;
     const size_t lds = ((S + 32 + 15) & ~15u) + 4ull * S;
-    hipLaunchKernelGGL(k_parse, dim3((uint32_t)nt1), dim3(kWG), lds, stream, d_stream, len, g, S, W,
+    SZP_LAUNCH("k_parse", k_parse, dim3((uint32_t)nt1), dim3(kWG), lds, stream, d_stream, len, g, S, W,
                        w.ctab, w.exit_st, w.cnt_st, w.blkpos, w.ctl);
     const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
     if (vec)
-        hipLaunchKernelGGL(k_decode<true>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_stream,
+        SZP_LAUNCH("k_decode", k_decode<true>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_stream,
                            len, g, w.ctab, w.blkpos, d_out, w.dec_st, w.ctl);
     else
-        hipLaunchKernelGGL(k_decode<false>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_stream,
+        SZP_LAUNCH("k_decode", k_decode<false>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_stream,
                            len, g, w.ctab, w.blkpos, d_out, w.dec_st, w.ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
