@@ -4,8 +4,12 @@
 Workload: a 512^3 float32 Gaussian random field with isotropic power spectrum
 proportional to k^-3, normalised to [-1, 1] (synthetic, seeded), compressed
 with a relative error bound of 1e-4 and block size 128 into T = 16 chunks.
-One step = one compress of the field plus one decompress of its stream, with
-the field already resident in HBM.  Variants (REL 1e-2, B = 32) are measured
+One step = one compress of the field (stream plus the codec's side index of
+block positions) plus one decompress of its stream using that index, with the
+field already resident in HBM.  Decompressing the same bytes as a foreign
+stream (no index: block boundaries recovered by the speculative parse) and
+compressing without the index are timed separately and reported under
+"paths".  Variants (REL 1e-2, B = 32) are measured
 after the main line and reported beside it.
 
 value = 2 * 4N / step time: uncompressed bytes through the codec per second
@@ -78,49 +82,57 @@ def load_reference(threads):
     return kind, comp, decomp
 
 
+def _timed(torch, fn, steps):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def measure(torch, szp, codec, x, rel, steps, warmup, profile):
+    """Step = compress (stream + side index) then decompress with the index.
+    Also times, separately: compress without the index, and decompress of the
+    same bytes as a foreign stream (block boundaries recovered by parsing)."""
     n = x.numel()
-    out, size_t = codec.compress(x, szp.REL, 0.0, rel)
+    out, size_t, idx = codec.compress(x, szp.REL, 0.0, rel, index=True)
     codec.check()
     nbytes = int(size_t.item())
     dec = torch.empty(n, dtype=torch.float32, device=x.device)
     for _ in range(warmup):
-        codec.compress(x, szp.REL, 0.0, rel, out=out)
+        codec.compress(x, szp.REL, 0.0, rel, out=out, index=idx)
+        codec.decompress(out, nbytes, n, out=dec, index=idx)
         codec.decompress(out, nbytes, n, out=dec)
     codec.check()
     if int(size_t.item()) != nbytes:
         raise RuntimeError("compressed size changed between runs")
 
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    def step():
+        codec.compress(x, szp.REL, 0.0, rel, out=out, index=idx)
+        codec.decompress(out, nbytes, n, out=dec, index=idx)
+
     if profile:
         szp.profile_enable(True)
-    e0.record()
-    for _ in range(steps):
-        codec.compress(x, szp.REL, 0.0, rel, out=out)
-        codec.decompress(out, nbytes, n, out=dec)
-    e1.record()
-    torch.cuda.synchronize()
-    step_ms = e0.elapsed_time(e1) / steps
+    step_ms = _timed(torch, step, steps)
     kernels = szp.profile_read() if profile else {}
     if profile:
         szp.profile_enable(False)
-
-    e0.record()
-    for _ in range(steps):
-        codec.compress(x, szp.REL, 0.0, rel, out=out)
-    e1.record()
-    torch.cuda.synchronize()
-    comp_ms = e0.elapsed_time(e1) / steps
-    e0.record()
-    for _ in range(steps):
-        codec.decompress(out, nbytes, n, out=dec)
-    e1.record()
-    torch.cuda.synchronize()
-    dec_ms = e0.elapsed_time(e1) / steps
+    comp_ms = _timed(torch, lambda: codec.compress(x, szp.REL, 0.0, rel, out=out, index=idx), steps)
+    dec_ms = _timed(torch, lambda: codec.decompress(out, nbytes, n, out=dec, index=idx), steps)
+    comp_plain_ms = _timed(torch, lambda: codec.compress(x, szp.REL, 0.0, rel, out=out), steps)
+    if profile:
+        szp.profile_enable(True)
+    dec_parse_ms = _timed(torch, lambda: codec.decompress(out, nbytes, n, out=dec), steps)
+    parse_kernels = szp.profile_read() if profile else {}
+    if profile:
+        szp.profile_enable(False)
     codec.check()
     return dict(out=out, nbytes=nbytes, dec=dec, step_ms=step_ms, comp_ms=comp_ms, dec_ms=dec_ms,
-                kernels=kernels)
+                comp_plain_ms=comp_plain_ms, dec_parse_ms=dec_parse_ms, kernels=kernels,
+                parse_kernels=parse_kernels, index_bytes=codec.index_size(n))
 
 
 def eb_report(torch, x, dec, stream_head):
@@ -175,8 +187,10 @@ def main():
     rep = eb_report(torch, x, r["dec"], stream_host)
 
     # Dominant kernel and its roofline (HBM-bound integer work; no MFMA).
-    alg = {"k_compress": 4 * N + C, "k_decode": C + 4 * N, "k_parse": C, "k_range_partial": 4 * N}
+    alg = {"k_compress": 4 * N + C, "k_decode_indexed": C + 4 * N, "k_decode": C + 4 * N,
+           "k_parse": C, "k_range_partial": 4 * N}
     kern = {k: v[1] / v[0] for k, v in r["kernels"].items() if v[0]}
+    pkern = {k: v[1] / v[0] for k, v in r["parse_kernels"].items() if v[0]}
     dom = max((k for k in kern if k in alg), key=lambda k: kern[k])
     achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
     workload = "grf512_k3_rel%g_B%d_T%d" % (rel, B, T)
@@ -184,7 +198,8 @@ def main():
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBPS, 4),
                     traffic=traffic_for(dom, workload), alg_bytes_per_launch=alg[dom],
                     avg_launch_ms=round(kern[dom], 5),
-                    kernels_avg_ms={k: round(v, 5) for k, v in sorted(kern.items())})
+                    kernels_avg_ms={k: round(v, 5) for k, v in sorted(kern.items())},
+                    foreign_decompress_kernels_avg_ms={k: round(v, 5) for k, v in sorted(pkern.items())})
 
     comp_gbps = 4 * N / (r["comp_ms"] * 1e-3) / 1e9
     dec_gbps = 4 * N / (r["dec_ms"] * 1e-3) / 1e9
@@ -228,6 +243,7 @@ def main():
             v = dict(rel=vrel, B=vB, ratio=round(4 * N / vr["nbytes"], 4),
                      compress_GBps=round(4 * N / (vr["comp_ms"] * 1e-3) / 1e9, 1),
                      decompress_GBps=round(4 * N / (vr["dec_ms"] * 1e-3) / 1e9, 1),
+                     decompress_foreign_stream_GBps=round(4 * N / (vr["dec_parse_ms"] * 1e-3) / 1e9, 1),
                      compress_hbm_frac=round((8 * N + vr["nbytes"]) / (vr["comp_ms"] * 1e-3) / 8e12, 4),
                      decompress_hbm_frac=round((vr["nbytes"] + 4 * N) / (vr["dec_ms"] * 1e-3) / 8e12, 4))
             if not args.no_cpu:
@@ -258,6 +274,13 @@ def main():
         "decompress_GBps": round(dec_gbps, 1),
         "compress_hbm_frac": round((8 * N + C) / (r["comp_ms"] * 1e-3) / 8e12, 4),
         "decompress_hbm_frac": round((C + 4 * N) / (r["dec_ms"] * 1e-3) / 8e12, 4),
+        "paths": {
+            "step": "compress writing stream + side index, then decompress using the index",
+            "side_index_bytes": r["index_bytes"],
+            "compress_without_index_GBps": round(4 * N / (r["comp_plain_ms"] * 1e-3) / 1e9, 1),
+            "decompress_foreign_stream_GBps": round(4 * N / (r["dec_parse_ms"] * 1e-3) / 1e9, 1),
+            "decompress_foreign_stream_hbm_frac": round((C + 4 * N) / (r["dec_parse_ms"] * 1e-3) / 8e12, 4),
+        },
         "compressed_bytes": C,
         "ratio": round(4 * N / C, 4),
         "error_bound": rep,

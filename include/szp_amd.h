@@ -60,6 +60,25 @@ int szp_amd_decompress_dev(const unsigned char *d_stream, uint64_t streamBytes, 
                            int blockSize, int nbThreads, float *d_out, void *d_work,
                            uint64_t workBytes, void *stream);
 
+/* Side index (no reference counterpart; the stream bytes are unchanged):
+ * the stream position of every block and, per tile of the codec, the
+ * quantized value preceding the tile.  The reference's format has no block
+ * index, so a decoder must otherwise recover block boundaries by parsing
+ * (szp_amd_decompress_dev does).  The *_indexed pair writes and consumes it;
+ * the index is only valid for the stream it was written with. */
+uint64_t szp_amd_index_size(uint64_t nbEle, int blockSize, int nbThreads);
+
+int szp_amd_compress_dev_indexed(const float *d_data, uint64_t nbEle, int errBoundMode,
+                                 double absErrBound, double relBoundRatio, int blockSize,
+                                 int nbThreads, unsigned char *d_out, uint64_t outCapacity,
+                                 uint64_t *d_outSize, void *d_index, uint64_t indexBytes,
+                                 void *d_work, uint64_t workBytes, void *stream);
+
+int szp_amd_decompress_dev_indexed(const unsigned char *d_stream, uint64_t streamBytes,
+                                   const void *d_index, uint64_t indexBytes, uint64_t nbEle,
+                                   int blockSize, int nbThreads, float *d_out, void *d_work,
+                                   uint64_t workBytes, void *stream);
+
 /* Synchronises `stream` and returns the status accumulated in d_work. */
 int szp_amd_check_status(void *d_work, void *stream);
 

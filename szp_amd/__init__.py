@@ -75,6 +75,10 @@ def _load():
         "szp_amd_bit_width": (i32, [ctypes.c_uint32]),
         "szp_amd_status_string": (ctypes.c_char_p, [i32]),
         "szp_amd_last_error": (ctypes.c_char_p, []),
+        "szp_amd_index_size": (u64, [u64, i32, i32]),
+        "szp_amd_compress_dev_indexed": (i32, [vp, u64, i32, dbl, dbl, i32, i32, vp, u64, vp, vp,
+                                               u64, vp, u64, vp]),
+        "szp_amd_decompress_dev_indexed": (i32, [vp, u64, vp, u64, u64, i32, i32, vp, vp, u64, vp]),
         "szp_amd_profile_enable": (None, [i32]),
         "szp_amd_profile_count": (i32, []),
         "szp_amd_profile_entry": (i32, [i32, ctypes.c_char_p, i32, ctypes.POINTER(u64),
@@ -207,9 +211,11 @@ class DeviceCodec:
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         return ctypes.c_void_p(s.cuda_stream)
 
-    def compress(self, x, err_mode=REL, abs_eb=0.0, rel=1e-4, out=None, stream=None):
-        """Returns (stream_tensor uint8[capacity], size_tensor int64[1]) ; the
-        valid stream is stream_tensor[:size]."""
+    def compress(self, x, err_mode=REL, abs_eb=0.0, rel=1e-4, out=None, stream=None, index=None):
+        """Returns (stream_tensor uint8[capacity], size_tensor int64[1]); the
+        valid stream is stream_tensor[:size].  With index=True (or a uint8
+        tensor of index_size bytes) it also writes the side index and returns
+        (stream_tensor, size_tensor, index_tensor)."""
         torch = self.torch
         assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous()
         n = x.numel()
@@ -220,14 +226,32 @@ class DeviceCodec:
         size = self._buf("csize", 256).view(torch.int64)
         work = self._buf("cwork", wsb)
         self.last_work = work
-        rc = lib.szp_amd_compress_dev(x.data_ptr(), n, int(err_mode), float(abs_eb), float(rel),
-                                      self.B, self.T, out.data_ptr(), out.numel(), size.data_ptr(),
-                                      work.data_ptr(), work.numel(), self._stream(stream))
+        if index is None or index is False:
+            rc = lib.szp_amd_compress_dev(x.data_ptr(), n, int(err_mode), float(abs_eb), float(rel),
+                                          self.B, self.T, out.data_ptr(), out.numel(),
+                                          size.data_ptr(), work.data_ptr(), work.numel(),
+                                          self._stream(stream))
+            if rc:
+                raise SzpError("szp_amd_compress_dev", rc)
+            return out, size[:1]
+        ib = self.index_size(n)
+        idx = self._buf("cidx", ib) if index is True else index
+        rc = lib.szp_amd_compress_dev_indexed(x.data_ptr(), n, int(err_mode), float(abs_eb),
+                                              float(rel), self.B, self.T, out.data_ptr(),
+                                              out.numel(), size.data_ptr(), idx.data_ptr(),
+                                              idx.numel(), work.data_ptr(), work.numel(),
+                                              self._stream(stream))
         if rc:
-            raise SzpError("szp_amd_compress_dev", rc)
-        return out, size[:1]
+            raise SzpError("szp_amd_compress_dev_indexed", rc)
+        return out, size[:1], idx
 
-    def decompress(self, stream_t, stream_bytes, n, out=None, stream=None):
+    def index_size(self, n):
+        return int(lib.szp_amd_index_size(int(n), self.B, self.T))
+
+    def decompress(self, stream_t, stream_bytes, n, out=None, stream=None, index=None):
+        """Decodes stream_t[:stream_bytes] (stream_bytes may be any upper bound
+        inside the tensor).  With `index` (from compress(index=...)) the block
+        boundaries and carries come from the side index instead of a parse."""
         torch = self.torch
         assert stream_t.is_cuda and stream_t.dtype == torch.uint8
         wsb = lib.szp_amd_decompress_workspace_size(int(stream_bytes), int(n), self.B, self.T)
@@ -235,11 +259,19 @@ class DeviceCodec:
         self.last_work = work
         if out is None:
             out = torch.empty(int(n), dtype=torch.float32, device=self.device)
-        rc = lib.szp_amd_decompress_dev(stream_t.data_ptr(), int(stream_bytes), int(n), self.B,
-                                        self.T, out.data_ptr(), work.data_ptr(), work.numel(),
-                                        self._stream(stream))
+        if index is None:
+            rc = lib.szp_amd_decompress_dev(stream_t.data_ptr(), int(stream_bytes), int(n), self.B,
+                                            self.T, out.data_ptr(), work.data_ptr(), work.numel(),
+                                            self._stream(stream))
+            if rc:
+                raise SzpError("szp_amd_decompress_dev", rc)
+            return out
+        rc = lib.szp_amd_decompress_dev_indexed(stream_t.data_ptr(), int(stream_bytes),
+                                                index.data_ptr(), index.numel(), int(n), self.B,
+                                                self.T, out.data_ptr(), work.data_ptr(),
+                                                work.numel(), self._stream(stream))
         if rc:
-            raise SzpError("szp_amd_decompress_dev", rc)
+            raise SzpError("szp_amd_decompress_dev_indexed", rc)
         return out
 
     def check(self, stream=None):

@@ -282,6 +282,32 @@ int szp_amd_decompress_dev(const unsigned char *d_stream, uint64_t streamBytes, 
                              workBytes, (hipStream_t)stream);
 }
 
+uint64_t szp_amd_index_size(uint64_t nbEle, int blockSize, int nbThreads)
+{
+    return index_bytes(nbEle, blockSize, nbThreads);
+}
+
+int szp_amd_compress_dev_indexed(const float *d_data, uint64_t nbEle, int errBoundMode,
+                                 double absErrBound, double relBoundRatio, int blockSize,
+                                 int nbThreads, unsigned char *d_out, uint64_t outCapacity,
+                                 uint64_t *d_outSize, void *d_index, uint64_t indexBytes,
+                                 void *d_work, uint64_t workBytes, void *stream)
+{
+    if (!d_index) return SZP_AMD_ERR_ARG;
+    return launch_compress(d_data, nbEle, errBoundMode, absErrBound, relBoundRatio, blockSize,
+                           nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
+                           (hipStream_t)stream, d_index, indexBytes);
+}
+
+int szp_amd_decompress_dev_indexed(const unsigned char *d_stream, uint64_t streamBytes,
+                                   const void *d_index, uint64_t indexBytes, uint64_t nbEle,
+                                   int blockSize, int nbThreads, float *d_out, void *d_work,
+                                   uint64_t workBytes, void *stream)
+{
+    return launch_decompress_indexed(d_stream, streamBytes, d_index, indexBytes, nbEle, blockSize,
+                                     nbThreads, d_out, d_work, workBytes, (hipStream_t)stream);
+}
+
 int szp_amd_check_status(void *d_work, void *stream)
 {
     if (!d_work) return SZP_AMD_ERR_ARG;

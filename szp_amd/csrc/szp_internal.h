@@ -52,9 +52,18 @@ enum : uint32_t {
 
 // Asynchronous launches on `stream`; all buffers are device memory.
 // Return 0 or a positive szp_amd_status code for argument errors.
+// d_index (optional, 256-byte aligned, index_bytes() long) receives the side
+// index: each block's stream position and each tile's carry.
 int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, double rel,
                     int B, int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size,
-                    void *d_work, uint64_t work_bytes, hipStream_t stream);
+                    void *d_work, uint64_t work_bytes, hipStream_t stream,
+                    void *d_index = nullptr, uint64_t idx_bytes = 0);
+
+uint64_t index_bytes(uint64_t n, int B, int T);
+
+int launch_decompress_indexed(const uint8_t *d_stream, uint64_t stream_bytes, const void *d_index,
+                              uint64_t idx_bytes, uint64_t n, int B, int T, float *d_out,
+                              void *d_work, uint64_t work_bytes, hipStream_t stream);
 
 int launch_decompress(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int B, int T,
                       float *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream);

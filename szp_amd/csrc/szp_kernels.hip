@@ -598,14 +598,24 @@ __device__ __forceinline__ void encode_octet(uint8_t *ob, uint32_t ib, const Oct
 // offset is known.  One decoupled look-back per tile places the tile; tiles
 // are ordered by blockIdx.x (each XCD dispatches its workgroups in index
 // order, and every wait is bounded).
+//
+// With INDEX the kernel also writes a side index beside the stream (the
+// stream itself is unchanged): the stream position of every block
+// (idx_pos[global block]) and, per tile, the q of the element before the
+// tile's first element (idx_carry[tile]), so that a decoder needs neither a
+// parse nor a scan across tiles.
+template <bool INDEX>
 __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
                                                   const float *__restrict__ d_eb, float eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
                                                   uint64_t *__restrict__ status,
                                                   uint32_t *__restrict__ ctl,
-                                                  uint64_t *__restrict__ d_size)
+                                                  uint64_t *__restrict__ d_size,
+                                                  uint64_t *__restrict__ idx_pos,
+                                                  uint32_t *__restrict__ idx_carry)
 {
     __shared__ __align__(16) uint8_t ob[kOutCap + 64];
+    __shared__ uint32_t s_ioff[INDEX ? kWG * kMaxRounds : 1];
     __shared__ uint32_t s_max[kWG];
     __shared__ uint32_t s_off[kWG];
     __shared__ uint32_t s_tot;
@@ -627,6 +637,10 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     uint32_t ct;
     uint64_t cj;
     locate(g, tI0, ct, cj);
+    const uint32_t t_first = ct;
+    const uint64_t j_first = cj;
+    if (INDEX && tid == 0)
+        idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
     uint32_t nit = (uint32_t)min<uint64_t>(g.K, tIN - tI0);
     Oct cur = octet_of(g, ct, cj, nit, d, o);
     OctIn cin;
@@ -691,6 +705,7 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
             s_head_off[k] = run + ib;
         }
         if (tid == 0) s_rbase[r] = run;
+        if (INDEX && o == 0 && cur.live) s_ioff[r * g.K + d] = run + ib;
         if (!deferring && run + rt <= (uint32_t)kOutCap) {
             encode_octet(ob, run + ib, cur, o, b, mg, sgn, qh);
             nbuf = r + 1;
@@ -715,6 +730,15 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     if (G0 + run > cap || (ld_ctl(&ctl[1]) & kErrTimeout)) {
         if (tid == 0 && G0 + run > cap) raise_error(ctl, kErrCapacity);
         return;
+    }
+    if (INDEX) {
+        const uint32_t nti = (uint32_t)(tIN - tI0);
+        for (uint32_t k = tid; k < nti; k += kWG) {
+            uint32_t t = t_first;
+            uint64_t j = j_first;
+            advance(g, t, j, k);
+            if (j) idx_pos[(uint64_t)t * g.nb0 + (j - 1)] = G0 + s_ioff[k];
+        }
     }
     // Chunk offsets relative to the body (SZp/src/szp_float.cc:280-288).
     for (uint32_t k = tid; k < s_nheads; k += kWG) {
@@ -1196,10 +1220,13 @@ __device__ __forceinline__ void load_bytes(const uint8_t *__restrict__ st, uint6
 // its segmented delta sum; one look-back gives the tile's carry (the q of the
 // element before the tile); pass 2 unpacks again and writes
 // fl32((float)q * eb).
-template <bool VEC>
+//
+// With INDEXED, blkpos and idx_carry come from the compressor's side index:
+// the carry is known up front, so the look-back and pass 1 are skipped.
+template <bool VEC, bool INDEXED>
 __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
-                                                Geom g, const uint64_t *__restrict__ ctab,
-                                                const uint64_t *__restrict__ blkpos,
+                                                Geom g, const uint64_t *__restrict__ blkpos,
+                                                const uint32_t *__restrict__ idx_carry,
                                                 float *__restrict__ out,
                                                 uint64_t *__restrict__ status,
                                                 uint32_t *__restrict__ ctl)
@@ -1208,7 +1235,6 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     __shared__ __align__(16) float fs[2048 + 8];
     __shared__ uint32_t s_pos[kWG * kMaxRounds];  // item positions - tile base
     __shared__ uint32_t s_wh[kWG / 64], s_wv[kWG / 64];
-    __shared__ uint32_t s_rxh[kMaxRounds], s_rxv[kMaxRounds];
     __shared__ uint64_t s_base, s_end;
     __shared__ uint64_t s_rE[kMaxRounds + 1];  // first element of each round
     __shared__ uint32_t s_carry;
@@ -1218,7 +1244,7 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     const uint64_t tile = blockIdx.x;
     if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
         // Successors may wait on this tile: publish before leaving.
-        if (tid == 0) st_status(&status[tile], kFlagIncl);
+        if (!INDEXED && tid == 0) st_status(&status[tile], kFlagIncl);
         return;
     }
     const uint32_t ebu = ((uint32_t)st[0] << 24) | ((uint32_t)st[1] << 16) |
@@ -1248,7 +1274,9 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
             uint32_t t = t0;
             uint64_t j = j0;
             advance(g, t, j, it);
-            mypos[k] = j == 0 ? ctab[t] : blkpos[(uint64_t)t * g.nb0 + (j - 1)];
+            // Chunk heads sit at the offsets of the stream header.
+            mypos[k] = j == 0 ? g.body + read_u64_le4(st + 4 + 8ull * t)
+                              : blkpos[(uint64_t)t * g.nb0 + (j - 1)];
             if (it == 0) s_base = mypos[k];
             if (it == nti - 1) {
                 uint64_t e;
@@ -1275,7 +1303,10 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
         }
     }
     if (whole) load_bytes(st, len, s_base, tend, buf);
-    if (tid == 0) s_bad = 0;
+    if (tid == 0) {
+        s_bad = 0;
+        if (INDEXED) s_carry = idx_carry[tile];
+    }
     __syncthreads();
     if (bad) s_bad = 1;
     __syncthreads();
@@ -1284,7 +1315,7 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     const uint32_t d = ((uint32_t)tid * g.spb_magic) >> 16;
     const uint32_t o = (uint32_t)tid - d * g.spb;
 
-    for (int pass = 0; pass < 2 && !s_bad; ++pass) {
+    for (int pass = INDEXED ? 1 : 0; pass < 2 && !s_bad; ++pass) {
         uint32_t th = 0, tv = 0;
         uint32_t carry = 0;
         if (pass == 1) carry = s_carry;
@@ -1318,20 +1349,14 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
             }
             uint32_t dl[8], h = 0, m = 0;
             bool b2 = false;
-            (void)0;
             if (active) m = decode_slot(L, j == 0, o, buf, s_pos[rfirst + d] - rb, rend - rb, dl, h, b2);
             if (b2) s_bad = 1;
             const uint32_t v = m ? (h ? dl[0] : dl[m - 1]) : 0u;
             uint32_t eh, ev, rh, rv;
             seg_scan_round(h, v, s_wh, s_wv, eh, ev, rh, rv);
-            if (pass == 0) {
-                if (tid == 0) {
-                    s_rxh[r] = th;
-                    s_rxv[r] = tv;
-                }
-            } else {
+            if (pass == 1) {
                 // carry (+) tile prefix of the round (+) slot prefix in the round
-                uint32_t before = s_rxh[r] ? s_rxv[r] : carry + s_rxv[r];
+                uint32_t before = th ? tv : carry + tv;
                 before = eh ? ev : before + ev;
                 const uint64_t E0 = s_rE[r], E1 = s_rE[r + 1];
                 const uint64_t Eb = E0 & ~3ull;
@@ -1406,7 +1431,7 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     __syncthreads();
     if (s_bad && tid == 0) {
         // Publish (harmlessly, if already published) so no successor waits.
-        st_status(&status[tile], kFlagIncl);
+        if (!INDEXED) st_status(&status[tile], kFlagIncl);
         raise_error(ctl, kErrStream);
     }
 }
@@ -1633,6 +1658,21 @@ uint64_t decompress_ws_layout(const Geom &g, uint64_t len, uint8_t *base, Decomp
 
 }  // namespace
 
+uint64_t index_bytes(uint64_t n, int B, int T)
+{
+    Geom g;
+    if (!make_geom(n, B, T, &g)) return 0;
+    const uint64_t nblocks = (uint64_t)(g.T - 1) * g.nb0 + g.nblast;
+    return align_up(8 * (nblocks ? nblocks : 1)) + align_up(4 * (g.ntiles ? g.ntiles : 1));
+}
+
+static void index_layout(const Geom &g, uint8_t *base, uint64_t **pos, uint32_t **carry)
+{
+    const uint64_t nblocks = (uint64_t)(g.T - 1) * g.nb0 + g.nblast;
+    *pos = reinterpret_cast<uint64_t *>(base);
+    *carry = reinterpret_cast<uint32_t *>(base + align_up(8 * (nblocks ? nblocks : 1)));
+}
+
 uint64_t compress_workspace_bytes(uint64_t n, int B, int T)
 {
     Geom g;
@@ -1657,7 +1697,7 @@ enum { kOk = 0, kArg = 1, kCap = 2, kHip = 3 };
 
 int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, double rel, int B,
                     int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size, void *d_work,
-                    uint64_t work_bytes, hipStream_t stream)
+                    uint64_t work_bytes, hipStream_t stream, void *d_index, uint64_t idx_bytes)
 {
     Geom g;
     if (!make_geom(n, B, T, &g) || !d_out || !d_size || !d_work) return kArg;
@@ -1667,6 +1707,8 @@ int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, d
     if ((reinterpret_cast<uintptr_t>(d_out) & 15) || (reinterpret_cast<uintptr_t>(d_work) & 255))
         return kArg;
     if (cap < g.body) return kCap;
+    if (d_index && ((reinterpret_cast<uintptr_t>(d_index) & 255) || idx_bytes < index_bytes(n, B, T)))
+        return d_index && idx_bytes < index_bytes(n, B, T) ? kCap : kArg;
     CompressWs w;
     if (compress_ws_layout(g, reinterpret_cast<uint8_t *>(d_work), &w) > work_bytes) return kCap;
 
@@ -1691,9 +1733,15 @@ int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, d
     if (n == 0) {
         SZP_LAUNCH("k_empty_stream", k_empty_stream, dim3(1), dim3(kWG), 0, stream, d_out, d_eb, eb_host,
                            g.T, d_size);
+    } else if (d_index) {
+        uint64_t *ipos;
+        uint32_t *icarry;
+        index_layout(g, reinterpret_cast<uint8_t *>(d_index), &ipos, &icarry);
+        SZP_LAUNCH("k_compress", k_compress<true>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream,
+                   d_x, d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size, ipos, icarry);
     } else {
-        SZP_LAUNCH("k_compress", k_compress, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_x,
-                   d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size);
+        SZP_LAUNCH("k_compress", k_compress<false>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream,
+                   d_x, d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size, nullptr, nullptr);
     }
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
@@ -1721,11 +1769,39 @@ int launch_decompress(const uint8_t *d_stream, uint64_t len, uint64_t n, int B, 
                        w.ctab, w.exit_st, w.cnt_st, w.blkpos, w.ctl);
     const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
     if (vec)
-        SZP_LAUNCH("k_decode", k_decode<true>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_stream,
-                           len, g, w.ctab, w.blkpos, d_out, w.dec_st, w.ctl);
+        SZP_LAUNCH("k_decode", (k_decode<true, false>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
+                   stream, d_stream, len, g, w.blkpos, nullptr, d_out, w.dec_st, w.ctl);
     else
-        SZP_LAUNCH("k_decode", k_decode<false>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream, d_stream,
-                           len, g, w.ctab, w.blkpos, d_out, w.dec_st, w.ctl);
+        SZP_LAUNCH("k_decode", (k_decode<false, false>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
+                   stream, d_stream, len, g, w.blkpos, nullptr, d_out, w.dec_st, w.ctl);
+    return hipGetLastError() == hipSuccess ? kOk : kHip;
+}
+
+int launch_decompress_indexed(const uint8_t *d_stream, uint64_t len, const void *d_index,
+                              uint64_t idx_bytes, uint64_t n, int B, int T, float *d_out,
+                              void *d_work, uint64_t work_bytes, hipStream_t stream)
+{
+    Geom g;
+    if (!make_geom(n, B, T, &g) || !d_stream || !d_work || !d_index) return kArg;
+    if (n && !d_out) return kArg;
+    if ((reinterpret_cast<uintptr_t>(d_stream) & 15) || (reinterpret_cast<uintptr_t>(d_work) & 255) ||
+        (reinterpret_cast<uintptr_t>(d_index) & 255))
+        return kArg;
+    if (idx_bytes < index_bytes(n, B, T) || work_bytes < 256) return kCap;
+    if (len < g.body) return kArg;
+    uint32_t *ctl = reinterpret_cast<uint32_t *>(d_work);
+    SZP_LAUNCH("k_clear", k_clear, dim3(1), dim3(kWG), 0, stream, ctl, nullptr, 0);
+    if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;
+    uint64_t *ipos;
+    uint32_t *icarry;
+    index_layout(g, reinterpret_cast<uint8_t *>(const_cast<void *>(d_index)), &ipos, &icarry);
+    const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+    if (vec)
+        SZP_LAUNCH("k_decode_indexed", (k_decode<true, true>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
+                   stream, d_stream, len, g, ipos, icarry, d_out, nullptr, ctl);
+    else
+        SZP_LAUNCH("k_decode_indexed", (k_decode<false, true>), dim3((uint32_t)g.ntiles), dim3(kWG),
+                   0, stream, d_stream, len, g, ipos, icarry, d_out, nullptr, ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 

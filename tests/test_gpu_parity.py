@@ -140,3 +140,36 @@ def test_decompress_accepts_upper_bound_length(szp, torch_mod):
     codec.check()
     ref = oracle_lib.decompress(want, x.size, B, T)
     assert np.array_equal(dec.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+INDEXED_CASES = [
+    ("waves", (1 << 20) + 13, 128, 16, 1, 0.0, 1e-4),
+    ("waves", 1 << 20, 32, 8, 1, 0.0, 1e-2),
+    ("zeroblocks", 500003, 64, 3, 1, 0.0, 1e-3),
+    ("walk", 70000, 9, 5, 0, 0.01, 0.0),
+    ("walk", 40, 8, 16, 0, 0.01, 0.0),
+    ("specials", 200000, 32, 4, 0, 0.5, 0.0),
+    ("uniform", 300000, 2048, 2, 0, 0.002, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", INDEXED_CASES, ids=lambda c: "%s-n%d-B%d-T%d" % (c[0], c[1], c[2], c[3]))
+def test_side_index_roundtrip(szp, torch_mod, case):
+    """compress(index=True) leaves the stream byte-identical; decompress with
+    the side index is bit-identical to the oracle's decompression."""
+    torch = torch_mod
+    kind, n, B, T, mode, ab, rel = case
+    x = make_input(kind, n, 11 + n)
+    say("indexed case", case)
+    want = oracle_lib.compress(x, mode, ab, rel, B, T)
+    codec = szp.DeviceCodec(block_size=B, threads=T)
+    xd = torch.from_numpy(x).cuda()
+    out, size_t, idx = codec.compress(xd, mode, ab, rel, index=True)
+    codec.check()
+    size = int(size_t.item())
+    assert size == want.size
+    assert np.array_equal(out[:size].cpu().numpy(), want)
+    dec = codec.decompress(out, size, n, index=idx)
+    codec.check()
+    ref = oracle_lib.decompress(want, n, B, T)
+    assert np.array_equal(dec.cpu().numpy().view(np.uint32), ref.view(np.uint32))
