@@ -1,0 +1,108 @@
+"""CPU checks of the product library's C ABI (no kernel launches): it loads,
+exports every entry point include/*.h declares, and its host-side helpers
+agree with the oracle and with the format."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "szp_amd", "libszp_amd.so")
+
+
+def declared_functions():
+    names = set()
+    for h in ("szp.h", "szp_amd.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            if name.startswith("szp_") and not name.endswith("_H"):
+                names.add(name)
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    assert os.path.exists(LIB), "build first: python szp_amd/build.py"
+    return ctypes.CDLL(LIB)
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = sorted(names - exported)
+    assert not missing, missing
+    for data in ("szp_versionNumber", "szp_dataEndianType", "szp_sysEndianType"):
+        assert data in exported
+    for n in names:
+        getattr(lib, n)
+
+
+def test_package_imports():
+    import szp_amd
+    assert szp_amd.lib is not None
+    assert szp_amd.SZP_NONRANDOMACCESS == 1 and szp_amd.SZ_FLOAT == 0
+    assert szp_amd.ABS == 0 and szp_amd.REL == 1
+
+
+def test_bit_width_matches_oracle():
+    import szp_amd
+    O = oracle_lib.oracle()
+    rng = np.random.default_rng(0)
+    probes = [0, 1, 2, 3, 255, 256, 2097150, 2097151, 4294964352, 4294964353, 0x80000000,
+              0xFFFFFFFF] + list(rng.integers(0, 1 << 32, 20000, dtype=np.uint64))
+    for m in probes:
+        assert szp_amd.bit_width(int(m)) == O.szpo_bit_width(int(m)), m
+
+
+def test_compress_bound_matches_oracle():
+    import szp_amd
+    O = oracle_lib.oracle()
+    for n in (0, 1, 2, 7, 100, 4097, 1 << 20, (1 << 27) + 5):
+        for B in (1, 8, 32, 128, 1000, 2048):
+            for T in (1, 3, 16):
+                assert szp_amd.compress_bound(n, B, T) == O.szpo_compress_bound(n, B, T)
+    assert szp_amd.compress_bound(100, 2049, 1) == 0  # block sizes above 2048 are rejected
+
+
+def test_stream_length_from_header(golden):
+    import szp_amd
+    cases, arrays, _ = golden
+    for c in cases:
+        s = arrays["stream_%d" % c["id"]]
+        assert szp_amd.stream_length(s, c["n"], c["B"], c["T"]) == s.size
+
+
+def test_threads_override():
+    import szp_amd
+    szp_amd.set_threads(5)
+    assert szp_amd.default_threads() == 5
+    szp_amd.set_threads(0)
+    assert szp_amd.default_threads() >= 1
+
+
+def test_unsupported_modes_return_null_without_a_device(lib):
+    import szp_amd
+    x = np.zeros(16, np.float32)
+    with pytest.raises(szp_amd.SzpError):
+        szp_amd.szp_compress(2, 0, x, 0, 0.1, 0.0, 16, 8)  # random-access mode
+    with pytest.raises(szp_amd.SzpError):
+        szp_amd.szp_compress(1, 1, x, 0, 0.1, 0.0, 16, 8)  # double
+    assert b"supported" in szp_amd.lib.szp_amd_last_error()
+
+
+def test_sizes_are_consistent():
+    import szp_amd
+    L = szp_amd.lib
+    for n, B, T in ((1000, 32, 2), (1 << 27, 128, 16), (5, 8, 16)):
+        assert L.szp_amd_compress_workspace_size(n, B, T) > 0
+        assert L.szp_amd_decompress_workspace_size(szp_amd.compress_bound(n, B, T), n, B, T) > 0
+        assert L.szp_amd_index_size(n, B, T) >= 8 * ((n + B - 1) // B)
