@@ -4,40 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "szp_geom.h"
+
 namespace szp_amd {
-
-constexpr int kWG = 256;                 // threads per workgroup (4 wave64)
-constexpr int kMaxBlock = 2048;          // largest SZp block size handled on the GPU
-constexpr int kOutBufBytes = 8704 + 64;  // LDS bytes for one round's encoded blocks
-constexpr int kMaxRounds = 8;            // rounds of K items per tile
-constexpr int kOutCap = 32768;           // LDS bytes buffering a compress tile before placement
-constexpr int kDecTileBytes = 24576;     // LDS bytes holding a decode tile's stream bytes
-
-// Geometry of the item sequence of one field.  The stream body is the
-// concatenation, in element order, of "items": the raw head value of each
-// non-empty chunk (4 bytes) followed by that chunk's SZp blocks.  A tile is K
-// consecutive items; every item owns spb = ceil(B/8) thread slots ("octets").
-struct Geom {
-    uint64_t n;        // elements
-    uint64_t tbs;      // floor(n / T), the size of chunks 0..T-2
-    uint64_t c0;       // items in chunks 0..T-2 (0 when tbs == 0)
-    uint64_t clast;    // items in chunk T-1
-    uint64_t nitems;   // (T-1)*c0 + clast
-    uint64_t nb0;      // SZp blocks in chunks 0..T-2
-    uint64_t nblast;   // SZp blocks in chunk T-1
-    uint64_t ntiles;   // ceil(nitems / K)
-    uint64_t body;     // 4 + 8T: offset of chunk 0 in the stream
-    uint32_t T;        // chunks (the reference's OpenMP team size)
-    uint32_t B;        // SZp block size
-    uint32_t spb;      // slots per item
-    uint32_t K;        // items per tile
-    uint32_t spb_magic;  // slot / spb == (slot * spb_magic) >> 16 for slot < 256
-    uint32_t R;          // rounds per tile (1 when chunks are smaller than a tile)
-};
-
-// Host-side construction; returns false when (n, B, T) is outside what the
-// kernels handle (B > kMaxBlock, T < 1).
-bool make_geom(uint64_t n, int B, int T, Geom *g);
 
 uint64_t compress_bound(uint64_t n, int B, int T);
 uint64_t compress_workspace_bytes(uint64_t n, int B, int T);

@@ -71,15 +71,9 @@ int bit_width_host(uint32_t m)
 __device__ __forceinline__ uint32_t bit_width(uint32_t m)
 {
     if (!m) return 0;
-    uint32_t k = 32u - __clz(m);
+    const uint32_t k = 32u - __clz(m);
+    if (__builtin_expect(k < 21, 1)) return k;
     return k + (m >= c_width_bump[k] ? 1u : 0u);
-}
-
-// Bytes of one block of L values at width b (SZp/src/szp_TypeManager.cc:20-31).
-__host__ __device__ __forceinline__ uint32_t block_bytes(uint32_t b, uint32_t L)
-{
-    if (!b) return 1;
-    return 1 + ((L + 7) >> 3) + (b >> 3) * L + (((b & 7) * L + 7) >> 3);
 }
 
 // (int)((double)x * (double)inv) with x86 cvttsd2si semantics: NaN and
@@ -184,63 +178,6 @@ __device__ uint64_t lookback_sum(uint64_t *st, uint64_t tile, uint64_t first, ui
     }
     if (lane == 0) st_status(&st[tile], kFlagIncl | (excl + agg));
     return excl;
-}
-
-// ---------------------------------------------------------------------------
-// Item geometry
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void locate(const Geom &g, uint64_t item, uint32_t &t, uint64_t &j)
-{
-    uint64_t lead = (uint64_t)(g.T - 1) * g.c0;
-    if (item < lead) {
-        t = (uint32_t)(item / g.c0);
-        j = item - (uint64_t)t * g.c0;
-    } else {
-        t = g.T - 1;
-        j = item - lead;
-    }
-}
-
-__device__ __forceinline__ void advance(const Geom &g, uint32_t &t, uint64_t &j, uint32_t d)
-{
-    j += d;
-    while (t < g.T - 1 && j >= g.c0) {
-        j -= g.c0;
-        ++t;
-    }
-}
-
-__host__ __device__ __forceinline__ uint64_t chunk_lo(const Geom &g, uint32_t t)
-{
-    return (uint64_t)t * g.tbs;
-}
-
-__host__ __device__ __forceinline__ uint64_t chunk_len(const Geom &g, uint32_t t)
-{
-    return t == g.T - 1 ? g.n - chunk_lo(g, t) : g.tbs;
-}
-
-__device__ __forceinline__ uint64_t item_elem(const Geom &g, uint32_t t, uint64_t j)
-{
-    return chunk_lo(g, t) + (j ? 1 + (j - 1) * (uint64_t)g.B : 0);
-}
-
-__device__ __forceinline__ uint32_t item_len(const Geom &g, uint32_t t, uint64_t j)
-{
-    if (!j) return 1;
-    uint64_t rem = chunk_len(g, t) - 1 - (j - 1) * (uint64_t)g.B;
-    return rem < g.B ? (uint32_t)rem : g.B;
-}
-
-// First element of item `item` (n for item == nitems).
-__device__ __forceinline__ uint64_t item_start(const Geom &g, uint64_t item)
-{
-    if (item >= g.nitems) return g.n;
-    uint32_t t;
-    uint64_t j;
-    locate(g, item, t, j);
-    return item_elem(g, t, j);
 }
 
 // Copies tot bytes from LDS to out[G0, G0 + tot): 16-byte stores for the
@@ -452,39 +389,6 @@ __device__ __forceinline__ void lds_st_bytes(uint8_t *p, uint64_t w, uint32_t n)
     if (n & 1) *p = (uint8_t)w;
 }
 
-// One thread's octet of a round: up to 8 consecutive elements of one item.
-struct Oct {
-    uint64_t e0;  // first element
-    uint32_t m;   // elements in the octet (0: nothing to do)
-    uint32_t L;   // elements of the item
-    uint32_t t;   // chunk of the item
-    bool live;    // the slot's item exists in this round
-    bool head;    // the item is a chunk head
-};
-
-__device__ __forceinline__ Oct octet_of(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nit,
-                                        uint32_t d, uint32_t o)
-{
-    Oct s{0, 0, 0, 0, false, false};
-    if (d >= g.K || d >= nit) return s;
-    uint32_t t = ct;
-    uint64_t j = cj;
-    advance(g, t, j, d);
-    s.live = true;
-    s.t = t;
-    s.head = j == 0;
-    s.L = item_len(g, t, j);
-    const uint64_t es = item_elem(g, t, j);
-    if (s.head) {
-        s.e0 = es;
-        s.m = o == 0 ? 1u : 0u;
-    } else {
-        s.e0 = es + 8ull * o;
-        s.m = 8 * o < s.L ? min(8u, s.L - 8 * o) : 0u;
-    }
-    return s;
-}
-
 // v[0]: the element before the octet (blocks only); v[1..8]: the octet.
 struct OctIn {
     float v[9];
@@ -591,6 +495,116 @@ __device__ __forceinline__ void encode_octet(uint8_t *ob, uint32_t ib, const Oct
     }
 }
 
+// Full-octet deltas for the simple-tile path: p = element before the octet,
+// a, b = the octet.  mag = max(d, -d) is exact for every int32 d, INT32_MIN
+// included (it maps to 0x80000000 as the reference's unsigned negation does).
+__device__ __forceinline__ uint32_t octet8_deltas(float p, const float4 &a, const float4 &b,
+                                                  float inv, uint32_t mg[8], uint32_t &sgn)
+{
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    int32_t prev = quantize(p, inv);
+    uint32_t omax = 0, sg = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int32_t c = quantize(v[i], inv);
+        const int32_t dd = (int32_t)((uint32_t)c - (uint32_t)prev);
+        prev = c;
+        const uint32_t mag = (uint32_t)max(dd, (int32_t)(0u - (uint32_t)dd));
+        mg[i] = mag;
+        sg = (sg << 1) | ((uint32_t)dd >> 31);
+        omax = max(omax, mag);
+    }
+    sgn = sg;
+    return omax;
+}
+
+// Encodes a full octet (8 values) of a block of L values at width b > 0.
+__device__ __forceinline__ void encode_octet8(uint8_t *ob, uint32_t ib, uint32_t o, uint32_t L,
+                                              uint32_t b, const uint32_t mg[8], uint32_t sgn)
+{
+    const uint32_t S8 = (L + 7) >> 3;
+    ob[ib + 1 + o] = (uint8_t)sgn;
+    const uint32_t nb = b >> 3, r = b & 7;
+    uint8_t *p1 = ob + ib + 1 + S8 + 8 * o * nb;
+    if (nb == 1) {
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = mg[i] >> r;
+        lds_st64(p1, v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24),
+                 v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24));
+    } else if (nb == 2) {
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = mg[i] >> r;
+        lds_st128(p1, v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16),
+                  v[6] | (v[7] << 16));
+    } else if (nb == 3) {
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = mg[i] >> r;
+        lds_st128(p1, v[0] | (v[1] << 24), (v[1] >> 8) | (v[2] << 16), (v[2] >> 16) | (v[3] << 8),
+                  v[4] | (v[5] << 24));
+        lds_st64(p1 + 16, (v[5] >> 8) | (v[6] << 16), (v[6] >> 16) | (v[7] << 8));
+    } else if (nb == 4) {
+        lds_st128(p1, mg[0], mg[1], mg[2], mg[3]);
+        lds_st128(p1 + 16, mg[4], mg[5], mg[6], mg[7]);
+    }
+    if (r) {
+        // 4r <= 28 bits per half: pack the residual stream in two 32-bit words.
+        const uint32_t mask = (1u << r) - 1;
+        uint32_t h = 0, l = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            h = (h << r) | (mg[i] & mask);
+            l = (l << r) | (mg[i + 4] & mask);
+        }
+        const uint64_t A = ((uint64_t)h << (4 * r)) | l;  // 8r bits, first value on top
+        const uint64_t w = __builtin_bswap64(A << (64 - 8 * r));
+        lds_st_bytes(ob + ib + 1 + S8 + nb * L + o * r, w, r);
+    }
+}
+
+// Byte lengths of a round's items and their exclusive offsets (s_off),
+// after every slot has folded its octet maximum into s_max.  Returns the
+// round's bytes.  Called by all threads; three barriers.
+__device__ __forceinline__ uint32_t round_layout(bool live, bool head, uint32_t L, uint32_t d,
+                                                 uint32_t o, uint32_t K, uint32_t *s_max,
+                                                 uint32_t *s_off, uint32_t *s_tot, uint32_t &b)
+{
+    const int tid = threadIdx.x;
+    // The scan below reads all kWG entries; items past the round add 0.  The
+    // previous round's readers of s_off finished before its closing barrier.
+    s_off[tid] = 0;
+    __syncthreads();
+    uint32_t len = 0;
+    b = 0;
+    if (live) {
+        if (head) {
+            len = 4;
+        } else {
+            b = bit_width(s_max[d]);
+            len = block_bytes(b, L);
+        }
+    }
+    if (o == 0 && d < K) s_off[d] = len;
+    __syncthreads();
+    s_max[tid] = 0;
+    if (tid < 64) {
+        const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
+        const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
+        const uint32_t sum = a0 + a1 + a2 + a3;
+        const uint32_t incl = wave_incl_scan_u32(sum, tid);
+        const uint32_t ex = incl - sum;
+        s_off[4 * tid] = ex;
+        s_off[4 * tid + 1] = ex + a0;
+        s_off[4 * tid + 2] = ex + a0 + a1;
+        s_off[4 * tid + 3] = ex + a0 + a1 + a2;
+        if (tid == 63) *s_tot = incl;
+    }
+    __syncthreads();
+    return *s_tot;
+}
+
 // A tile is g.R rounds of K items.  Each thread owns one octet per round and
 // loads it straight from HBM one round ahead of its use.  Rounds are encoded
 // into the LDS buffer ob (kOutCap bytes) as long as they fit; the first round
@@ -647,9 +661,63 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     octet_load(x, cur, cin);
     __syncthreads();
 
-    uint32_t run = 0;   // tile-relative byte offset of the current round
+    uint64_t run = 0;   // tile-relative byte offset of the current round
     uint32_t nbuf = 0;  // rounds held in ob
     bool deferring = false;
+    // A simple tile holds only full blocks of one chunk (B % 8 == 0): every
+    // octet has 8 values at a 32-bit offset from the tile's first element.
+    const uint32_t nti_all = (uint32_t)(tIN - tI0);
+    if (simple_tile(g, ct, cj, nti_all)) {
+        // Element indices are 64-bit: Et >= 1, so Et + off - 1 never wraps.
+        const uint64_t Et = item_elem(g, ct, cj);
+        const uint32_t B = g.B;
+        uint32_t off = simple_octet_off(g, 0, d, o);
+        bool live = d < g.K && d < nti_all;
+        float cp = 0.f, np_ = 0.f;
+        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca, na = ca, nb = ca;
+        if (live) {
+            cp = x[Et + off - 1];
+            __builtin_memcpy(&ca, x + Et + off, 16);
+            __builtin_memcpy(&cb, x + Et + off + 4, 16);
+        }
+        for (uint32_t r = 0; r < g.R; ++r) {
+            const uint32_t first = r * g.K;
+            if (first >= nti_all) break;
+            const uint32_t noff = simple_octet_off(g, r + 1, d, o);
+            const bool nlive = d < g.K && first + g.K + d < nti_all;
+            if (nlive) {
+                np_ = x[Et + noff - 1];
+                __builtin_memcpy(&na, x + Et + noff, 16);
+                __builtin_memcpy(&nb, x + Et + noff + 4, 16);
+            }
+            uint32_t mg[8], sgn = 0;
+            if (live) {
+                const uint32_t omax = octet8_deltas(cp, ca, cb, inv, mg, sgn);
+                if (omax) atomicMax(&s_max[d], omax);
+            }
+            uint32_t b;
+            const uint32_t rt = round_layout(live, false, B, d, o, g.K, s_max, s_off, &s_tot, b);
+            const uint32_t ib = live ? s_off[d] : 0;
+            if (tid == 0) s_rbase[r] = (uint32_t)run;
+            if (INDEX && o == 0 && live) s_ioff[first + d] = (uint32_t)run + ib;
+            if (!deferring && run + rt <= (uint64_t)kOutCap) {
+                if (live) {
+                    if (o == 0) ob[run + ib] = (uint8_t)b;
+                    if (b) encode_octet8(ob, run + ib, o, B, b, mg, sgn);
+                }
+                nbuf = r + 1;
+            } else {
+                deferring = true;
+            }
+            run += rt;
+            off = noff;
+            live = nlive;
+            cp = np_;
+            ca = na;
+            cb = nb;
+            __syncthreads();
+        }
+    } else
     for (uint32_t r = 0; r < g.R; ++r) {
         const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
         if (ri0 >= tIN) break;
@@ -702,11 +770,11 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
         if (cur.head && o == 0) {
             const uint32_t k = atomicAdd(&s_nheads, 1u);
             s_head_t[k] = cur.t;
-            s_head_off[k] = run + ib;
+            s_head_off[k] = (uint32_t)run + ib;
         }
-        if (tid == 0) s_rbase[r] = run;
-        if (INDEX && o == 0 && cur.live) s_ioff[r * g.K + d] = run + ib;
-        if (!deferring && run + rt <= (uint32_t)kOutCap) {
+        if (tid == 0) s_rbase[r] = (uint32_t)run;
+        if (INDEX && o == 0 && cur.live) s_ioff[r * g.K + d] = (uint32_t)run + ib;
+        if (!deferring && run + rt <= (uint64_t)kOutCap) {
             encode_octet(ob, run + ib, cur, o, b, mg, sgn, qh);
             nbuf = r + 1;
         } else {
@@ -765,7 +833,7 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
         }
     }
     const bool all_buffered = tI0 + (uint64_t)nbuf * g.K >= tIN;
-    const uint32_t buffered = all_buffered ? run : s_rbase[nbuf];
+    const uint32_t buffered = all_buffered ? (uint32_t)run : s_rbase[nbuf];
     copy_out_bytes(out, G0, buffered, ob, tid);
     // Rounds that did not fit: encode again and write them one at a time.
     for (uint32_t r = nbuf; r < g.R; ++r) {
@@ -812,6 +880,13 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
         __syncthreads();
         encode_octet(ob, s.live ? s_off[d] : 0, s, o, b, mg, sgn, qh);
         __syncthreads();
+        // The round was sized once already; a different size here would be a
+        // defect, so it is reported instead of written.
+        const uint64_t rend = (ri0 + g.K < tIN) ? (uint64_t)s_rbase[r + 1] : run;
+        if ((uint64_t)s_rbase[r] + s_tot != rend || s_tot > (uint32_t)kOutCap) {
+            if (tid == 0) raise_error(ctl, kErrCapacity);
+            return;
+        }
         copy_out_bytes(out, G0 + s_rbase[r], s_tot, ob, tid);
     }
 }
@@ -1439,34 +1514,6 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-
-bool make_geom(uint64_t n, int B, int T, Geom *g)
-{
-    if (B < 1 || B > kMaxBlock || T < 1) return false;
-    const uint64_t b = (uint64_t)B;
-    auto items_of = [&](uint64_t m) -> uint64_t { return m ? 1 + (m - 1 + b - 1) / b : 0; };
-    auto blocks_of = [&](uint64_t m) -> uint64_t { return m ? (m - 1 + b - 1) / b : 0; };
-    g->n = n;
-    g->T = (uint32_t)T;
-    g->B = (uint32_t)B;
-    g->tbs = n / (uint64_t)T;
-    const uint64_t nlast = n - (uint64_t)(T - 1) * g->tbs;
-    g->c0 = items_of(g->tbs);
-    g->clast = items_of(nlast);
-    g->nb0 = blocks_of(g->tbs);
-    g->nblast = blocks_of(nlast);
-    g->nitems = (uint64_t)(T - 1) * g->c0 + g->clast;
-    g->spb = (uint32_t)((B + 7) / 8);
-    g->K = (uint32_t)kWG / g->spb;
-    g->spb_magic = (65536u + g->spb - 1) / g->spb;
-    // Tiles of kMaxRounds rounds, unless chunks are so short that a tile would
-    // hold more chunk heads than one workgroup records (at most kWG).
-    g->R = (g->c0 == 0 || g->c0 * (kWG - 1) >= (uint64_t)g->K * kMaxRounds) ? kMaxRounds : 1;
-    const uint64_t per_tile = (uint64_t)g->K * g->R;
-    g->ntiles = (g->nitems + per_tile - 1) / per_tile;
-    g->body = 4 + 8ull * (uint64_t)T;
-    return true;
-}
 
 uint64_t compress_bound(uint64_t n, int B, int T)
 {

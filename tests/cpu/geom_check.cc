@@ -1,0 +1,131 @@
+// Host-side check of the kernels' index arithmetic: walks every tile, round
+// and thread slot of the compress and decode kernels (the same szp_geom.h
+// functions the kernels call) and asserts that every element a thread loads
+// lies inside the field and inside the thread's own item, and that the
+// octets of a round cover the round's elements exactly once.
+// Usage: geom_check            (built-in geometry list; prints a summary)
+#include <stdio.h>
+#include <stdlib.h>
+#include <stdint.h>
+#include <vector>
+#include <random>
+
+#include "szp_geom.h"
+
+using namespace szp_amd;
+
+static uint64_t g_checks = 0;
+static int g_fail = 0;
+
+#define CHECK(cond, ...)                                                          \
+    do {                                                                          \
+        ++g_checks;                                                               \
+        if (!(cond)) {                                                            \
+            if (g_fail < 20) {                                                    \
+                fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__);              \
+                fprintf(stderr, __VA_ARGS__);                                     \
+                fprintf(stderr, "\n");                                            \
+            }                                                                     \
+            ++g_fail;                                                             \
+        }                                                                         \
+    } while (0)
+
+// One compress tile: the element set each slot touches, round by round.
+static void check_compress_tile(const Geom &g, uint64_t tile, std::vector<uint8_t> *cover)
+{
+    const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
+    const uint64_t tIN = tI0 + (uint64_t)g.K * g.R < g.nitems ? tI0 + (uint64_t)g.K * g.R : g.nitems;
+    uint32_t ct;
+    uint64_t cj;
+    locate(g, tI0, ct, cj);
+    const uint32_t nti = (uint32_t)(tIN - tI0);
+    const bool simple = simple_tile(g, ct, cj, nti);
+    uint32_t rt = ct;
+    uint64_t rj = cj;
+    for (uint32_t r = 0; r < g.R; ++r) {
+        const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
+        if (ri0 >= tIN) break;
+        const uint32_t nit = (uint32_t)((tIN - ri0) < g.K ? tIN - ri0 : g.K);
+        // The deferred path recomputes the cursor with locate(); it must agree.
+        uint32_t lt;
+        uint64_t lj;
+        locate(g, ri0, lt, lj);
+        CHECK(lt == rt && lj == rj, "cursor mismatch tile %llu round %u", (unsigned long long)tile, r);
+        const uint64_t rE0 = item_elem(g, rt, rj), rE1 = item_start(g, ri0 + nit);
+        CHECK(rE1 - rE0 <= 2048, "round spans %llu elements", (unsigned long long)(rE1 - rE0));
+        for (uint32_t tid = 0; tid < (uint32_t)kWG; ++tid) {
+            const uint32_t d = (tid * g.spb_magic) >> 16;
+            const uint32_t o = tid - d * g.spb;
+            CHECK(d == tid / g.spb, "slot magic");
+            const Oct s = octet_of(g, rt, rj, nit, d, o);
+            if (simple) {
+                const uint64_t Et = item_elem(g, ct, cj);
+                const bool live = d < g.K && r * g.K + d < nti;
+                CHECK(live == s.live, "simple liveness differs from the general path");
+                if (!live) continue;
+                const uint32_t off = simple_octet_off(g, r, d, o);
+                CHECK(s.m == 8 && !s.head && s.e0 == Et + off,
+                      "simple octet (tile %llu r %u d %u o %u) off %u vs general e0 %llu m %u",
+                      (unsigned long long)tile, r, d, o, off, (unsigned long long)s.e0, s.m);
+                CHECK(Et + off >= 1 && Et + off + 8 <= g.n, "simple load out of field");
+            }
+            if (!s.m) continue;
+            CHECK(s.e0 + s.m <= g.n, "octet past the field end");
+            if (!s.head) CHECK(s.e0 >= 1, "block octet at element 0");
+            CHECK(s.e0 >= rE0 && s.e0 + s.m <= rE1, "octet outside its round");
+            if (cover) {
+                for (uint32_t i = 0; i < s.m; ++i) {
+                    uint8_t &c = (*cover)[s.e0 + i];
+                    CHECK(c == 0, "element %llu covered twice", (unsigned long long)(s.e0 + i));
+                    c = 1;
+                }
+            }
+        }
+        advance(g, rt, rj, nit);
+    }
+}
+
+static void check_geometry(uint64_t n, int B, int T, bool full)
+{
+    Geom g;
+    if (!make_geom(n, B, T, &g)) {
+        CHECK(false, "make_geom rejected n=%llu B=%d T=%d", (unsigned long long)n, B, T);
+        return;
+    }
+    CHECK((uint64_t)g.K * g.spb <= (uint64_t)kWG, "K * spb > kWG");
+    CHECK(g.K * (g.B + 0ull) <= 2048 || g.B > 2048, "round holds more than 2048 elements");
+    // Worst-case bytes of one round fit the decode round buffer.
+    CHECK((uint64_t)g.K * block_bytes(32, g.B) <= (uint64_t)kOutBufBytes, "round bytes > buffer");
+    std::vector<uint8_t> cover;
+    if (full) cover.assign(n, 0);
+    if (full) {
+        for (uint64_t t = 0; t < g.ntiles; ++t) check_compress_tile(g, t, &cover);
+        for (uint64_t e = 0; e < n; ++e) CHECK(cover[e] == 1, "element %llu not covered", (unsigned long long)e);
+    } else {
+        std::mt19937_64 rng(n ^ (uint64_t)B * 131 ^ (uint64_t)T);
+        std::vector<uint64_t> tiles;
+        for (uint64_t t = 0; t < g.ntiles && t < 300; ++t) tiles.push_back(t);
+        for (uint64_t t = g.ntiles > 300 ? g.ntiles - 300 : 0; t < g.ntiles; ++t) tiles.push_back(t);
+        for (int k = 0; k < 600 && g.ntiles; ++k) tiles.push_back(rng() % g.ntiles);
+        for (uint64_t t : tiles) check_compress_tile(g, t, nullptr);
+    }
+}
+
+int main()
+{
+    const int Bs[] = {1, 2, 3, 7, 8, 9, 16, 24, 31, 32, 64, 100, 127, 128, 255, 256, 512, 1000, 1024, 2048};
+    const int Ts[] = {1, 2, 3, 7, 8, 16};
+    const uint64_t ns[] = {1, 2, 7, 8, 9, 15, 16, 17, 40, 100, 1000, 4097, 20001, 65537, 70000, 300001};
+    for (uint64_t n : ns)
+        for (int B : Bs)
+            for (int T : Ts) check_geometry(n, B, T, true);
+    // Full-size shapes, sampled tiles.
+    check_geometry(134217728ull, 128, 16, false);
+    check_geometry(134217728ull, 32, 16, false);
+    check_geometry(67108864ull, 128, 8, false);
+    check_geometry(4294967296ull, 64, 16, false);
+    check_geometry(4294967296ull, 64, 1, false);
+    check_geometry(4194307ull, 128, 8, false);
+    printf("geom_check: %llu checks, %d failures\n", (unsigned long long)g_checks, g_fail);
+    return g_fail ? 1 : 0;
+}
