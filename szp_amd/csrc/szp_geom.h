@@ -4,6 +4,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <string.h>
 
 #ifdef __HIPCC__
 #define SZP_HD __host__ __device__ __forceinline__
@@ -178,6 +179,190 @@ SZP_HD Oct octet_of(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nit, uint3
         s.m = 8 * o < s.L ? (s.L - 8 * o < 8u ? s.L - 8 * o : 8u) : 0u;
     }
     return s;
+}
+
+// ---- bitmap helpers (one bit per stream byte of a parse tile) ----
+SZP_HD bool bm_test(const uint32_t *bm, uint32_t p)
+{
+    return (bm[p >> 5] >> (p & 31)) & 1u;
+}
+SZP_HD void bm_set(uint32_t *bm, uint32_t p) { bm[p >> 5] |= 1u << (p & 31); }
+SZP_HD void bm_clear(uint32_t *bm, uint32_t p) { bm[p >> 5] &= ~(1u << (p & 31)); }
+// Bits [p, p + k) as a mask of word w (k may be 0).
+SZP_HD uint32_t bm_mask(uint32_t w, uint32_t p, uint32_t k)
+{
+    const uint32_t lo = w << 5, a = p > lo ? p - lo : 0u, e = p + k - lo;
+    const uint32_t b = e < 32u ? e : 32u;
+    if (a >= b) return 0u;
+    const uint32_t hi = b == 32u ? 0xFFFFFFFFu : ((1u << b) - 1u);
+    return hi & ~((1u << a) - 1u);
+}
+SZP_HD void bm_set_range(uint32_t *bm, uint32_t p, uint32_t k)
+{
+    if (!k) return;
+    for (uint32_t w = p >> 5; w <= (p + k - 1) >> 5; ++w) bm[w] |= bm_mask(w, p, k);
+}
+SZP_HD void bm_clear_range(uint32_t *bm, uint32_t p, uint32_t k)
+{
+    if (!k) return;
+    for (uint32_t w = p >> 5; w <= (p + k - 1) >> 5; ++w) bm[w] &= ~bm_mask(w, p, k);
+}
+// First set bit in [p, p + k), or p + k.
+SZP_HD uint32_t bm_first(const uint32_t *bm, uint32_t p, uint32_t k)
+{
+    if (!k) return p;
+    for (uint32_t w = p >> 5; w <= (p + k - 1) >> 5; ++w) {
+        const uint32_t m = bm[w] & bm_mask(w, p, k);
+        if (m) return (w << 5) + __builtin_ctz(m);
+    }
+    return p + k;
+}
+
+// Zero bytes starting at buf[p], at most v - p (b == 0 blocks are single
+// zero bytes, so a run of k zeros on a chain is k consecutive blocks).
+// Reads up to 3 bytes past v, inside the buffer's slack.
+SZP_HD uint32_t zero_run(const uint8_t *buf, uint32_t p, uint32_t v)
+{
+    uint32_t k = 0;
+    while (p + k < v) {
+        uint32_t w;
+        __builtin_memcpy(&w, buf + p + k, 4);
+        if (w) {
+            k += (uint32_t)__builtin_ctz(w) >> 3;
+            break;
+        }
+        k += 4;
+    }
+    return k < v - p ? k : v - p;
+}
+
+// Length of a block with width byte b and B values.
+SZP_HD uint32_t blen(uint32_t b, uint32_t B) { return block_bytes(b, B); }
+
+}  // namespace szp_amd
+
+namespace szp_amd {
+
+// ---- per-thread walks of the block-boundary parse (used by k_parse and by
+// the host emulation in tests/cpu/parse_check.cc) ----
+//
+// tb[p] is byte p of the parse tile; [u, v) is one sub-tile; a chain's
+// positions are tile offsets.  spec/dead/pre are bitmaps over the tile.
+
+// Speculative chain of sub-tile [u, v): the first candidate c in
+// [u, min(u + W, v)) whose chain reaches v.  Its block starts are set in
+// spec; positions of failed chains are set in dead.  Returns true and the
+// block count and exit (>= v) on success.
+SZP_HD bool parse_spec_walk(const uint8_t *tb, uint32_t *spec, uint32_t *dead, uint32_t u,
+                            uint32_t v, uint32_t W, uint32_t B, uint32_t &cnt, uint32_t &exit)
+{
+    const uint32_t cmax = (u + W < v) ? u + W : v;
+    for (uint32_t c = u; c < cmax; ++c) {
+        if (bm_test(dead, c)) continue;
+        uint32_t p = c, n = 0;
+        bool died = false;
+        while (p < v) {
+            if (bm_test(dead, p)) { died = true; break; }
+            const uint32_t b = tb[p];
+            if (b > 32) { died = true; break; }
+            if (b == 0) {
+                const uint32_t k = zero_run(tb, p, v);
+                bm_set_range(spec, p, k);
+                n += k;
+                p += k;
+            } else {
+                bm_set(spec, p);
+                ++n;
+                p += blen(b, B);
+            }
+        }
+        if (!died) {
+            cnt = n;
+            exit = p;
+            return true;
+        }
+        // Move this attempt's marks from spec to dead: every position of a
+        // chain that dies inside the sub-tile leads to the same death.
+        const uint32_t stop = p;
+        uint32_t q = c;
+        while (q < stop) {
+            const uint32_t b = tb[q];
+            if (b == 0) {
+                uint32_t k = zero_run(tb, q, v);
+                if (k > stop - q) k = stop - q;
+                bm_clear_range(spec, q, k);
+                bm_set_range(dead, q, k);
+                q += k;
+            } else {
+                bm_clear(spec, q);
+                bm_set(dead, q);
+                q += blen(b, B);
+            }
+        }
+        if (stop < v) bm_set(dead, stop);
+    }
+    cnt = 0;
+    exit = v;
+    return false;
+}
+
+// Chain from candidate c: 0 if it joins spec or dies inside [u, v), else the
+// position (>= v) where it leaves.
+SZP_HD uint32_t parse_cand_walk(const uint8_t *tb, const uint32_t *spec, const uint32_t *dead,
+                                uint32_t c, uint32_t v, uint32_t B)
+{
+    uint32_t p = c;
+    while (true) {
+        if (bm_test(spec, p) || bm_test(dead, p)) return 0;
+        const uint32_t b = tb[p];
+        if (b > 32) return 0;
+        if (b == 0) {
+            const uint32_t k = zero_run(tb, p, v);
+            if (bm_first(spec, p, k) < p + k) return 0;
+            p += k;
+        } else {
+            p += blen(b, B);
+        }
+        if (p >= v) return p;
+    }
+}
+
+// True chain from p in [u, v): marks its blocks before it joins spec in pre.
+// Returns 1 merged (p = the join position), 2 broken (width byte > 32 at p),
+// 0 left the sub-tile (p >= v).  npre counts the marked blocks.
+SZP_HD int parse_true_walk(const uint8_t *tb, const uint32_t *spec, uint32_t *pre, uint32_t &p,
+                           uint32_t v, uint32_t B, uint32_t &npre)
+{
+    npre = 0;
+    while (p < v) {
+        if (bm_test(spec, p)) return 1;
+        const uint32_t b = tb[p];
+        if (b > 32) return 2;
+        if (b == 0) {
+            const uint32_t k = zero_run(tb, p, v);
+            const uint32_t q = bm_first(spec, p, k);
+            bm_set_range(pre, p, q - p);
+            npre += q - p;
+            if (q < p + k) {
+                p = q;
+                return 1;
+            }
+            p += k;
+        } else {
+            bm_set(pre, p);
+            ++npre;
+            p += blen(b, B);
+        }
+    }
+    return 0;
+}
+
+// Rank of p among the set bits of bm within its sub-tile (rk holds, per word,
+// the set bits of the sub-tile's earlier words).
+SZP_HD uint32_t parse_rank(const uint32_t *bm, const uint16_t *rk, uint32_t p)
+{
+    const uint32_t w = p >> 5;
+    return rk[w] + (uint32_t)__builtin_popcount(bm[w] & ((1u << (p & 31)) - 1u));
 }
 
 }  // namespace szp_amd

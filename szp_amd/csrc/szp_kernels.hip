@@ -946,18 +946,28 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
     ttab[g.T] = tiles;
 }
 
-// Recovers the start of every SZp block.  Parse tile i of chunk t covers
-// bytes [s, te) of the chunk's block area.  The first block starting at or
-// after s is the "entry"; the first at or after te is the "exit" that the
-// next tile takes as its entry.  A tile walks one speculative chain, then
-// checks that every chain starting in [s, s + W) (W = longest block) either
-// joins it, reaches an invalid width byte, or leaves the tile at the same
-// exit.  If so the exit does not depend on the entry and is published before
-// the predecessor finishes; otherwise the tile waits for its predecessor's
-// exit.  Block counts are then scanned with a decoupled look-back and every
-// block's stream position is written to blkpos[].
+// Recovers the start of every SZp block of a stream that carries no index.
+// Parse tile i of chunk t covers bytes [s, te) of the chunk's block area and
+// is split into four sub-tiles of U bytes, one per wave.  The "entry" of a
+// range is the first block starting in it, its "exit" the first block
+// starting at or after its end.
+//  1. Each wave walks one speculative chain through its sub-tile: the first
+//     candidate c in [u, u + W) (W = longest block) whose chain reaches the
+//     sub-tile end; failed chains are marked dead, so the search is linear.
+//     Runs of zero bytes (b = 0 blocks) are stepped over a word at a time.
+//  2. For the last sub-tile only, every chain starting in [u, u + W) must
+//     join the speculative chain, die on a width byte > 32 or leave at the
+//     same exit: then the tile's exit is known without its entry and is
+//     published before the predecessor tile finishes.
+//  3. One thread resolves the true chain: the tile's entry (the chunk's first
+//     block, or the predecessor's published exit) is walked until it joins
+//     the sub-tile's speculative chain, then the next sub-tile starts at that
+//     chain's exit.
+//  4. Block counts are scanned with a decoupled look-back over the chunk's
+//     tiles and every block's stream position is written to blkpos[].
+// A stream whose chains never synchronise is still parsed exactly, serially.
 __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, uint64_t len,
-                                               Geom g, uint32_t S, uint32_t W,
+                                               Geom g, uint32_t U, uint32_t W,
                                                const uint64_t *__restrict__ ctab,
                                                uint64_t *__restrict__ exit_st,
                                                uint64_t *__restrict__ cnt_st,
@@ -965,21 +975,22 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
                                                uint32_t *__restrict__ ctl)
 {
     extern __shared__ __align__(16) uint8_t dyn[];
+    const uint32_t S = 4 * U, NW = S / 32;
     uint8_t *buf = dyn;
-    uint16_t *rank = reinterpret_cast<uint16_t *>(dyn + ((S + 32 + 15) & ~15u));
-    uint16_t *lst = rank + S;
-    __shared__ uint32_t s_lenB[33];
-    __shared__ uint64_t s_tile, s_spec_exit, s_prefix, s_lastpos;
-    __shared__ uint32_t s_spec_cnt, s_npre, s_r0, s_cnt;
-    __shared__ int s_spec_ok, s_nonsync, s_broken, s_merged;
+    uint32_t *spec = reinterpret_cast<uint32_t *>(dyn + ((S + 64 + 15) & ~15u));
+    uint32_t *dead = spec + NW;
+    uint32_t *pre = dead + NW;
+    uint16_t *rspec = reinterpret_cast<uint16_t *>(pre + NW);  // spec bits before word, per sub-tile
+    uint16_t *rpre = rspec + NW;
+    __shared__ uint64_t s_exit[4];
+    __shared__ uint32_t s_cnt[4], s_npre[4], s_mrank[4];
+    __shared__ int s_merged[4], s_ok[4], s_broken[4];
+    __shared__ int s_nonsync;
+    __shared__ uint64_t s_prefix, s_lastpos;
 
-    const int tid = threadIdx.x;
-    if (tid == 0) s_tile = blockIdx.x;
-    if (tid < 33) s_lenB[tid] = block_bytes(tid, g.B);
-    if (tid == 0) { s_nonsync = 0; s_spec_ok = 0; s_spec_cnt = 0; s_spec_exit = 0; }
-    __syncthreads();
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    const uint64_t gt = blockIdx.x;
     const uint64_t *cstart = ctab, *aend = ctab + g.T, *ttab = ctab + 2 * g.T;
-    const uint64_t gt = s_tile;
     if (gt >= ttab[g.T]) return;
     uint32_t lo = 0, hi = g.T - 1;
     while (lo < hi) {
@@ -994,7 +1005,8 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     const uint64_t te = min(s + S, ae);
     const uint32_t span = (uint32_t)(te - s);
     const uint64_t s0 = s & ~15ull;
-    const uint32_t sh = (uint32_t)(s - s0);
+    const uint32_t sh = (uint32_t)(s - s0);  // buf[p + sh] holds tile byte p
+    const uint32_t B = g.B;
 
     for (uint64_t a = s0 + 16ull * tid; a < te; a += 16ull * kWG) {
         if (a + 16 <= len) {
@@ -1004,128 +1016,155 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
                 if (a + k < len) buf[a - s0 + k] = st[a + k];
         }
     }
-    for (uint32_t k = tid; k < span; k += kWG) rank[k] = 0;
-    __syncthreads();
-
-    if (i > 0) {
-        // One speculative chain: the first candidate in [s, s + W) whose walk
-        // reaches te.  Positions of failed walks are marked dead, so the
-        // search is linear in the tile size.
-        if (tid == 0) {
-            const uint32_t cmax = min(W, span);
-            int ok = 0;
-            uint32_t cnt = 0, p = 0;
-            for (uint32_t c = 0; c < cmax && !ok; ++c) {
-                if (rank[c] == kDead) continue;
-                p = c;
-                cnt = 0;
-                while (p < span) {
-                    const uint32_t b = buf[p + sh];
-                    if (b > 32 || rank[p] == kDead) break;
-                    lst[cnt++] = (uint16_t)p;
-                    p += s_lenB[b];
-                }
-                if (p >= span) ok = 1;
-                else for (uint32_t k = 0; k < cnt; ++k) rank[lst[k]] = kDead;
-            }
-            s_spec_ok = ok;
-            s_spec_cnt = ok ? cnt : 0;
-            s_spec_exit = s + p;
-            s_nonsync = !ok;
-        }
-        __syncthreads();
-        for (uint32_t k = tid; k < s_spec_cnt; k += kWG) rank[lst[k]] = (uint16_t)(k + 1);
-        __syncthreads();
-        if (s_spec_ok) {
-            const uint32_t cmax = min(W, span);
-            const uint64_t sx = s_spec_exit;
-            for (uint32_t c = tid; c < cmax; c += kWG) {
-                uint32_t p = c;
-                while (true) {
-                    const uint16_t rk = rank[p];
-                    if (rk) break;  // joins the chain, or is known dead
-                    const uint32_t b = buf[p + sh];
-                    if (b > 32) break;
-                    p += s_lenB[b];
-                    if (p >= span) {
-                        if (s + p != sx) s_nonsync = 1;
-                        break;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (tid == 0 && !s_nonsync) st_status(&exit_st[gt], kFlagIncl | s_spec_exit);
+    for (uint32_t k = tid; k < NW; k += kWG) {
+        spec[k] = 0;
+        dead[k] = 0;
+        pre[k] = 0;
     }
-
-    // The true chain from the entry.
     if (tid == 0) {
-        uint64_t e = as;
-        if (i > 0) e = wait_status(&exit_st[gt - 1], ctl) & kValMask;
-        uint32_t npre = 0, r0 = 0, cnt = 0;
-        int broken = 0, merged = 0;
-        uint64_t ex = te;
-        if (e < s) {
-            broken = 1;
-        } else {
-            uint64_t p = e;
-            // Dead marks only steer the speculation: past the true end of the
-            // last chunk the true chain may run into them, so they are
-            // stepped over here and only an invalid width byte stops it.
-            while (p < te) {
-                const uint16_t rk = rank[p - s];
-                if (rk && rk != kDead) { merged = 1; break; }
-                const uint32_t b = buf[p - s0];
-                if (b > 32) { broken = 1; break; }
-                lst[npre++] = (uint16_t)(p - s);
-                p += s_lenB[b];
-            }
-            if (broken) {
-                cnt = npre;
-                ex = te;
-            } else if (merged) {
-                r0 = rank[p - s] - 1;
-                cnt = npre + s_spec_cnt - r0;
-                ex = s_spec_exit;
-            } else {
-                cnt = npre;
-                ex = p;
-            }
-        }
-        if (i == 0 || s_nonsync) st_status(&exit_st[gt], kFlagIncl | ex);
-        s_npre = npre;
-        s_r0 = r0;
-        s_cnt = cnt;
-        s_broken = broken;
-        s_merged = merged;
+        s_nonsync = 0;
         s_lastpos = ~0ull;
     }
     __syncthreads();
-    if (tid < 64) {
-        const uint64_t pre = lookback_sum(cnt_st, gt, first, s_cnt, ctl);
-        if (tid == 0) s_prefix = pre;
+    const uint8_t *tb = buf + sh;  // tile byte p is tb[p]
+
+    // 1. Speculative chain of each sub-tile (lane 0 of wave w).
+    const uint32_t u = wv * U, v = min(u + U, span);
+    if (lane == 0) {
+        uint32_t cnt = 0, ex = v;
+        const bool ok = u < span && parse_spec_walk(tb, spec, dead, u, v, W, B, cnt, ex);
+        s_ok[wv] = ok;
+        s_cnt[wv] = ok ? cnt : 0;
+        s_exit[wv] = s + (ok ? ex : v);
     }
     __syncthreads();
 
-    const uint64_t pre = s_prefix;
-    const uint64_t nbt = (t == g.T - 1) ? g.nblast : g.nb0;
-    const uint64_t gbase = (uint64_t)t * g.nb0;
-    const uint32_t npre = s_npre, r0 = s_r0;
-    for (uint32_t k = tid; k < npre; k += kWG) {
-        const uint64_t bi = pre + k;
-        if (bi < nbt) {
-            blkpos[gbase + bi] = s + lst[k];
-            if (bi == nbt - 1) s_lastpos = s + lst[k];
+    // 2. Does the last sub-tile's exit depend on its entry?
+    const uint32_t lastw = span ? (span - 1) / U : 0;
+    if (wv == (int)lastw && span) {
+        if (!s_ok[lastw]) {
+            if (lane == 0) s_nonsync = 1;
+        } else {
+            const uint32_t cmax = min(u + W, v);
+            const uint64_t sx = s_exit[lastw];
+            for (uint32_t c = u + lane; c < cmax; c += 64) {
+                const uint32_t leave = parse_cand_walk(tb, spec, dead, c, v, B);
+                if (leave && s + leave != sx) s_nonsync = 1;
+            }
         }
     }
-    if (s_merged) {
-        for (uint32_t q = tid; q < span; q += kWG) {
-            const uint16_t rk = rank[q];
-            if (rk && rk != kDead && (uint32_t)(rk - 1) >= r0) {
-                const uint64_t bi = pre + npre + (rk - 1 - r0);
+    __syncthreads();
+    if (tid == 0 && !s_nonsync && span) st_status(&exit_st[gt], kFlagIncl | s_exit[lastw]);
+
+    // Spec ranks: spec bits before each word, within its sub-tile.
+    {
+        const uint32_t wpw = U / 32;  // words per sub-tile
+        const uint32_t w0 = wv * wpw;
+        uint32_t run = 0;
+        for (uint32_t k0 = 0; k0 < wpw; k0 += 64) {  // every lane runs every step
+            const uint32_t k = k0 + lane;
+            const uint32_t c = k < wpw ? (uint32_t)__builtin_popcount(spec[w0 + k]) : 0u;
+            const uint32_t incl = wave_incl_scan_u32(c, lane);
+            if (k < wpw) rspec[w0 + k] = (uint16_t)(run + incl - c);
+            run += __shfl(incl, 63);
+        }
+    }
+    __syncthreads();
+
+    // 3. True chain, sub-tile by sub-tile.
+    if (tid == 0) {
+        uint64_t e = as;
+        if (i > 0) e = wait_status(&exit_st[gt - 1], ctl) & kValMask;
+        bool broken = e < s;  // an exit before this tile: corrupt stream
+        for (uint32_t w = 0; w < 4; ++w) {
+            const uint32_t uw = w * U, vw = min(uw + U, span);
+            s_npre[w] = 0;
+            s_merged[w] = 0;
+            s_mrank[w] = 0;
+            s_broken[w] = broken;
+            if (uw >= span || broken || e >= s + vw) continue;  // no block starts here
+            uint32_t p = (uint32_t)(e - s), npre = 0;
+            const int res = parse_true_walk(tb, spec, pre, p, vw, B, npre);
+            s_npre[w] = npre;
+            if (res == 1) {
+                s_merged[w] = 1;
+                s_mrank[w] = parse_rank(spec, rspec, p);
+                e = s_ok[w] ? s_exit[w] : s + vw;
+            } else if (res == 2) {
+                broken = true;
+                s_broken[w] = 1;
+            } else {
+                e = s + p;
+            }
+        }
+        if (s_nonsync && span) st_status(&exit_st[gt], kFlagIncl | (broken ? te : e));
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < 4; ++w) {
+            const uint32_t c = s_npre[w] + (s_merged[w] ? s_cnt[w] - s_mrank[w] : 0u);
+            s_cnt[w] = c;  // blocks of sub-tile w on the true chain
+            total += c;
+        }
+        s_prefix = total;
+    }
+    __syncthreads();
+    const uint32_t tile_cnt = (uint32_t)s_prefix;
+    __syncthreads();
+    if (tid < 64) {
+        const uint64_t pre_cnt = lookback_sum(cnt_st, gt, first, tile_cnt, ctl);
+        if (tid == 0) s_prefix = pre_cnt;
+    }
+    // Pre ranks: pre bits before each word, within its sub-tile.
+    {
+        const uint32_t wpw = U / 32;
+        const uint32_t w0 = wv * wpw;
+        uint32_t run = 0;
+        for (uint32_t k0 = 0; k0 < wpw; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            const uint32_t c = k < wpw ? (uint32_t)__builtin_popcount(pre[w0 + k]) : 0u;
+            const uint32_t incl = wave_incl_scan_u32(c, lane);
+            if (k < wpw) rpre[w0 + k] = (uint16_t)(run + incl - c);
+            run += __shfl(incl, 63);
+        }
+    }
+    __syncthreads();
+
+    // 4. Block positions.
+    const uint64_t prefix = s_prefix;
+    const uint64_t nbt = (t == g.T - 1) ? g.nblast : g.nb0;
+    const uint64_t gbase = (uint64_t)t * g.nb0;
+    uint64_t wbase[4];
+    {
+        uint64_t acc = prefix;
+        for (int w = 0; w < 4; ++w) {
+            wbase[w] = acc;
+            acc += s_cnt[w];
+        }
+    }
+    for (uint32_t k = tid; k < NW; k += kWG) {
+        const uint32_t w = (k * 32) / U;
+        uint32_t m = pre[k];
+        while (m) {
+            const uint32_t bit = __builtin_ctz(m);
+            m &= m - 1;
+            const uint32_t p = k * 32 + bit;
+            const uint64_t bi = wbase[w] + parse_rank(pre, rpre, p);
+            if (bi < nbt) {
+                blkpos[gbase + bi] = s + p;
+                if (bi == nbt - 1) s_lastpos = s + p;
+            }
+        }
+        if (s_merged[w]) {
+            m = spec[k];
+            while (m) {
+                const uint32_t bit = __builtin_ctz(m);
+                m &= m - 1;
+                const uint32_t p = k * 32 + bit;
+                const uint32_t rk = parse_rank(spec, rspec, p);
+                if (rk < s_mrank[w]) continue;
+                const uint64_t bi = wbase[w] + s_npre[w] + (rk - s_mrank[w]);
                 if (bi < nbt) {
-                    blkpos[gbase + bi] = s + q;
-                    if (bi == nbt - 1) s_lastpos = s + q;
+                    blkpos[gbase + bi] = s + p;
+                    if (bi == nbt - 1) s_lastpos = s + p;
                 }
             }
         }
@@ -1133,8 +1172,9 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     __syncthreads();
     if (tid == 0) {
         bool bad = false;
-        if (s_broken && pre + npre < nbt) bad = true;
-        if (gt + 1 == ttab[t + 1] && pre + s_cnt < nbt) bad = true;  // truncated chunk
+        for (uint32_t w = 0; w < 4; ++w)
+            if (s_broken[w] && wbase[w] + s_npre[w] < nbt) bad = true;
+        if (gt + 1 == ttab[t + 1] && prefix + tile_cnt < nbt) bad = true;  // truncated chunk
         if (s_lastpos != ~0ull) {
             // The last block is shorter when (n_t - 1) % B != 0; its end must
             // be the chunk end (exact for all but the last chunk).
@@ -1672,11 +1712,21 @@ struct DecompressWs {
     uint64_t nclr;
 };
 
-uint32_t parse_tile_bytes(int B)
+// Sub-tile bytes of the parse: at least the longest block, so that a
+// sub-tile's entry always lies inside it.
+uint32_t parse_subtile_bytes(int B)
 {
     const uint32_t W = block_bytes(32, (uint32_t)B);
-    uint32_t S = W > 4096 ? W : 4096;
-    return (S + 255) & ~255u;
+    const uint32_t U = W > 8192 ? W : 8192;
+    return (U + 255) & ~255u;
+}
+
+uint32_t parse_tile_bytes(int B) { return 4 * parse_subtile_bytes(B); }
+
+size_t parse_lds_bytes(int B)
+{
+    const uint32_t S = parse_tile_bytes(B), NW = S / 32;
+    return ((S + 64 + 15) & ~15u) + 3ull * NW * 4 + 2ull * NW * 2;
 }
 
 uint64_t decompress_ws_layout(const Geom &g, uint64_t len, uint8_t *base, DecompressWs *w)
@@ -1811,9 +1861,9 @@ int launch_decompress(const uint8_t *d_stream, uint64_t len, uint64_t n, int B, 
                        S, w.ctab, w.exit_st, w.nclr, w.ctl);
     if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;
     const uint64_t nt1 = len / S + 1 + g.T;
-    const size_t lds = ((S + 32 + 15) & ~15u) + 4ull * S;
-    SZP_LAUNCH("k_parse", k_parse, dim3((uint32_t)nt1), dim3(kWG), lds, stream, d_stream, len, g, S, W,
-                       w.ctab, w.exit_st, w.cnt_st, w.blkpos, w.ctl);
+    SZP_LAUNCH("k_parse", k_parse, dim3((uint32_t)nt1), dim3(kWG), parse_lds_bytes(B), stream,
+               d_stream, len, g, parse_subtile_bytes(B), W, w.ctab, w.exit_st, w.cnt_st, w.blkpos,
+               w.ctl);
     const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
     if (vec)
         SZP_LAUNCH("k_decode", (k_decode<true, false>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
