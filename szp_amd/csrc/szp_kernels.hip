@@ -230,82 +230,77 @@ __global__ void k_clear(uint32_t *ctl, uint64_t *status, uint64_t nstatus)
 struct RangePart {
     float mn;
     float mx;
-    uint64_t imn;
-    uint64_t imx;
-    uint64_t pad;
 };
 
-// (v, i) beats (w, j) as a minimum: NaN never wins, the earlier index wins
-// ties.  This reproduces the serial scan "if (min > v) min = v" of
-// SZp/src/szp.cc:36-45 (first occurrence kept; only +0/-0 can tie with
-// different bits).
-__device__ __forceinline__ bool min_beats(float v, uint64_t i, float w, uint64_t j)
-{
-    return !isnan(v) && (isnan(w) || v < w || (v == w && i < j));
-}
-
-__device__ __forceinline__ bool max_beats(float v, uint64_t i, float w, uint64_t j)
-{
-    return !isnan(v) && (isnan(w) || v > w || (v == w && i < j));
-}
-
-__device__ void range_reduce_wg(float &mn, uint64_t &imn, float &mx, uint64_t &imx)
+// The serial scan of SZp/src/szp.cc:36-45 keeps the first occurrence of the
+// minimum and maximum, skips NaN (every comparison with it is false) and
+// returns x[0] for both when all values are equal.  Its result enters the
+// stream only through fl32(max - min), and that difference does not depend on
+// which of two equal values was kept: with max != min one side is nonzero and
+// the other's zero sign cannot change the exact difference, and with max ==
+// min the scan subtracts x[0] from itself, i.e. max - max.  So the GPU keeps
+// no indices: fminf/fmaxf (which return the non-NaN operand) in any order.
+__device__ __forceinline__ void range_reduce_wg(float &mn, float &mx)
 {
     __shared__ float smn[kWG / 64], smx[kWG / 64];
-    __shared__ uint64_t simn[kWG / 64], simx[kWG / 64];
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o; o >>= 1) {
-        float v = __shfl_xor(mn, o);
-        uint64_t i = __shfl_xor(imn, o);
-        if (min_beats(v, i, mn, imn)) { mn = v; imn = i; }
-        v = __shfl_xor(mx, o);
-        i = __shfl_xor(imx, o);
-        if (max_beats(v, i, mx, imx)) { mx = v; imx = i; }
+        mn = fminf(mn, __shfl_xor(mn, o));
+        mx = fmaxf(mx, __shfl_xor(mx, o));
     }
-    if (lane == 0) { smn[wv] = mn; simn[wv] = imn; smx[wv] = mx; simx[wv] = imx; }
+    if (lane == 0) {
+        smn[wv] = mn;
+        smx[wv] = mx;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < kWG / 64; ++k) {
-            if (min_beats(smn[k], simn[k], mn, imn)) { mn = smn[k]; imn = simn[k]; }
-            if (max_beats(smx[k], simx[k], mx, imx)) { mx = smx[k]; imx = simx[k]; }
+            mn = fminf(mn, smn[k]);
+            mx = fmaxf(mx, smx[k]);
         }
     }
 }
+
+// Min and max of x[s, s + seg) per workgroup.  VEC: x is 16-byte aligned and
+// seg a multiple of 4 * kWG * kRangeUnroll elements (only the last segment
+// is ragged).
+constexpr int kRangeUnroll = 4;
 
 template <bool VEC>
 __global__ __launch_bounds__(kWG) void k_range_partial(const float *__restrict__ x, uint64_t n,
                                                        uint64_t seg, RangePart *part)
 {
     const uint64_t s = (uint64_t)blockIdx.x * seg;
-    const uint64_t e = min(s + seg, n);
+    const uint32_t len = (uint32_t)(min(s + seg, n) - s);
+    const float *xs = x + s;
     float mn = __builtin_nanf(""), mx = __builtin_nanf("");
-    uint64_t imn = ~0ull, imx = ~0ull;
     if (VEC) {
-        // seg is a multiple of 4 and x is 16-byte aligned.
-        for (uint64_t v = s + 4ull * threadIdx.x; v < e; v += 4ull * kWG) {
-            float f[4];
-            if (v + 4 <= e) {
-                float4 q = *reinterpret_cast<const float4 *>(x + v);
-                f[0] = q.x; f[1] = q.y; f[2] = q.z; f[3] = q.w;
-            } else {
-                for (int k = 0; k < 4; ++k) f[k] = (v + k < e) ? x[v + k] : __builtin_nanf("");
-            }
+        constexpr uint32_t kStep = 4u * kWG * kRangeUnroll;
+        const uint32_t full = len / kStep * kStep;
+        for (uint32_t v = 4u * threadIdx.x; v < full; v += kStep) {
+            float4 q[kRangeUnroll];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (min_beats(f[k], v + k, mn, imn)) { mn = f[k]; imn = v + k; }
-                if (max_beats(f[k], v + k, mx, imx)) { mx = f[k]; imx = v + k; }
+            for (int u = 0; u < kRangeUnroll; ++u)
+                q[u] = *reinterpret_cast<const float4 *>(xs + v + 4u * kWG * u);
+#pragma unroll
+            for (int u = 0; u < kRangeUnroll; ++u) {
+                mn = fminf(mn, fminf(fminf(q[u].x, q[u].y), fminf(q[u].z, q[u].w)));
+                mx = fmaxf(mx, fmaxf(fmaxf(q[u].x, q[u].y), fmaxf(q[u].z, q[u].w)));
             }
+        }
+        for (uint32_t i = full + threadIdx.x; i < len; i += kWG) {
+            mn = fminf(mn, xs[i]);
+            mx = fmaxf(mx, xs[i]);
         }
     } else {
-        for (uint64_t i = s + threadIdx.x; i < e; i += kWG) {
-            float f = x[i];
-            if (min_beats(f, i, mn, imn)) { mn = f; imn = i; }
-            if (max_beats(f, i, mx, imx)) { mx = f; imx = i; }
+        for (uint32_t i = threadIdx.x; i < len; i += kWG) {
+            mn = fminf(mn, xs[i]);
+            mx = fmaxf(mx, xs[i]);
         }
     }
-    range_reduce_wg(mn, imn, mx, imx);
-    if (threadIdx.x == 0) part[blockIdx.x] = RangePart{mn, mx, imn, imx, 0};
+    range_reduce_wg(mn, mx);
+    if (threadIdx.x == 0) part[blockIdx.x] = RangePart{mn, mx};
 }
 
 // eb = fl32(fl32(max - min) * fl32(rel)), NaN when x[0] is NaN
@@ -315,19 +310,18 @@ __global__ __launch_bounds__(kWG) void k_range_final(const float *__restrict__ x
                                                      float relf, float *d_eb)
 {
     float mn = __builtin_nanf(""), mx = __builtin_nanf("");
-    uint64_t imn = ~0ull, imx = ~0ull;
     for (int k = threadIdx.x; k < nparts; k += kWG) {
-        RangePart p = part[k];
-        if (min_beats(p.mn, p.imn, mn, imn)) { mn = p.mn; imn = p.imn; }
-        if (max_beats(p.mx, p.imx, mx, imx)) { mx = p.mx; imx = p.imx; }
+        const RangePart p = part[k];
+        mn = fminf(mn, p.mn);
+        mx = fmaxf(mx, p.mx);
     }
-    range_reduce_wg(mn, imn, mx, imx);
+    range_reduce_wg(mn, mx);
     if (threadIdx.x == 0) {
         float eb;
         if (isnan(x[0])) {
             eb = x[0];
         } else {
-            float range = mx - mn;
+            const float range = (mx == mn) ? mx - mx : mx - mn;
             eb = range * relf;
         }
         *d_eb = eb;
@@ -1817,7 +1811,7 @@ int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, d
     const bool vec = (reinterpret_cast<uintptr_t>(d_x) & 15) == 0;
     if (err_mode == 1) {
         uint64_t seg = (n + kRangeParts - 1) / kRangeParts;
-        seg = (seg + 4095) & ~4095ull;
+        seg = (seg + 4095) & ~4095ull;  // a multiple of 4 * kWG * kRangeUnroll
         const int np = (int)((n + seg - 1) / seg);
         if (vec)
             SZP_LAUNCH("k_range_partial", k_range_partial<true>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);

@@ -65,6 +65,8 @@ def make_input(kind, n, seed):
         x = np.zeros(n, dtype=np.float32)
         x[rng.random(n) < 0.5] = np.float32(-0.0)
         return x
+    if kind == "posinf":
+        return np.full(n, np.inf, dtype=np.float32)
     if kind == "nanfirst":
         x = _walk(rng, n)
         x[0] = np.float32(np.nan)

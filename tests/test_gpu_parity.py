@@ -90,6 +90,9 @@ CASES = [
     ("tiny", 50000, 32, 2, 0, 5e-39, 0.0),
     ("signedzero", 50000, 32, 4, 1, 0.0, 1e-3),
     ("nanfirst", 20000, 32, 2, 1, 0.0, 1e-3),
+    ("specials", 300001, 32, 4, 1, 0.0, 1e-3),   # REL over +-Inf: eb = Inf
+    ("posinf", 10000, 32, 2, 1, 0.0, 1e-3),      # REL, all +Inf: eb = Inf - Inf = NaN
+    ("signedzero", (1 << 22) + 5, 128, 16, 1, 0.0, 1e-3),  # many range partials of +-0
 ]
 
 
