@@ -143,6 +143,19 @@ def eb_report(torch, x, dec, stream_head):
                 strict_over_eb=int((err > eb).sum()), within_rounding_bound=bool((err <= tol).all()))
 
 
+def copy_calibration(torch, steps):
+    """A plain copy of 512 MiB, 1 read + 1 write per byte: its FETCH_SIZE and
+    WRITE_SIZE readings per dispatch give the counters' scale."""
+    nbytes = 512 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ms = _timed(torch, lambda: dst.copy_(src), steps)
+    print(json.dumps({"copy_bytes": nbytes, "ms": round(ms, 5),
+                      "GBps_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}))
+
+
 def traffic_for(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -165,6 +178,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only time a plain 512 MiB device-to-device copy (the known-size kernel "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -174,6 +190,10 @@ def main():
     import numpy as np
     import torch
     import szp_amd as szp
+
+    if args.copy_calibration:
+        copy_calibration(torch, args.steps)
+        return
 
     n3, B, T, rel = args.n, args.block, args.threads, args.rel
     log("generating %d^3 k^-3 field" % n3)

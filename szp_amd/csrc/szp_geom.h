@@ -17,9 +17,9 @@ namespace szp_amd {
 constexpr int kWG = 256;                 // threads per workgroup (4 wave64)
 constexpr int kMaxBlock = 2048;          // largest SZp block size handled on the GPU
 constexpr int kOutBufBytes = 8704 + 64;  // LDS bytes for one round's encoded blocks
-constexpr int kMaxRounds = 8;            // rounds of K items per tile
-constexpr int kOutCap = 32768;           // LDS bytes buffering a compress tile before placement
-constexpr int kDecTileBytes = 24576;     // LDS bytes holding a decode tile's stream bytes
+constexpr int kMaxRounds = 4;            // rounds of K items per tile
+constexpr int kOutCap = 28672;           // LDS bytes buffering a compress tile before placement
+constexpr int kDecTileBytes = 20480;     // LDS bytes holding a decode tile's stream bytes
 
 // Geometry of the item sequence of one field.  The stream body is the
 // concatenation, in element order, of "items": the raw head value of each
@@ -150,6 +150,63 @@ SZP_HD uint32_t simple_octet_off(const Geom &g, uint32_t r, uint32_t d, uint32_t
 }
 
 // One thread's octet of a round: up to 8 consecutive elements of one item.
+// ---- dispatch order of the compress tiles ----
+// Tiles that are not simple (a chunk head, a chunk's short last block) take
+// the general path, which publishes its byte count for the look-back later
+// than a simple tile; every later tile waits for it.  They are therefore
+// dispatched first: workgroup b < n runs general tile t[b], the others run
+// the remaining tiles in increasing order.  n = 0 keeps blockIdx order.
+constexpr int kMaxGenTiles = 64;
+
+struct TileOrder {
+    uint32_t n;
+    uint32_t t[kMaxGenTiles];  // ascending
+};
+
+// Tiles holding a chunk head or a chunk's short last block (ascending, each
+// once).  n = 0 when there are more than kMaxGenTiles, or when every tile
+// takes the general path anyway (B % 8 != 0).
+inline TileOrder general_tiles(const Geom &g)
+{
+    TileOrder o;
+    o.n = 0;
+    if (g.B & 7) return o;
+    const uint64_t per_tile = (uint64_t)g.K * g.R;
+    auto add = [&](uint64_t tile) -> bool {
+        if (o.n && o.t[o.n - 1] == tile) return true;
+        if (o.n == (uint32_t)kMaxGenTiles) return false;
+        o.t[o.n++] = (uint32_t)tile;
+        return true;
+    };
+    for (uint32_t t = 0; t < g.T; ++t) {
+        const uint64_t s = (uint64_t)t * g.c0;
+        const uint64_t c = t + 1 < g.T ? g.c0 : g.clast;
+        if (!c) continue;
+        bool ok = add(s / per_tile);
+        if (ok && (chunk_len(g, t) - 1) % g.B) ok = add((s + c - 1) / per_tile);
+        if (!ok) {
+            o.n = 0;
+            return o;
+        }
+    }
+    return o;
+}
+
+// The tile that workgroup b runs.
+SZP_HD uint64_t tile_of_block(const TileOrder &o, uint64_t b)
+{
+    if (o.n == 0) return b;
+    if (b < o.n) return o.t[b];
+    // m-th tile outside the list: the first k with t[k] - k > m gives m + k.
+    const uint64_t m = b - o.n;
+    uint32_t lo = 0, hi = o.n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)o.t[mid] - mid > m) hi = mid; else lo = mid + 1;
+    }
+    return m + lo;
+}
+
 struct Oct {
     uint64_t e0;  // first element
     uint32_t m;   // elements in the octet (0: nothing to do)

@@ -150,18 +150,24 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane)
     return v;
 }
 
+// Publishes a tile's aggregate (its inclusive value for the first tile of
+// the scan domain) so that successors can look back past it.
+__device__ __forceinline__ void lookback_publish(uint64_t *st, uint64_t tile, uint64_t first,
+                                                 uint64_t agg)
+{
+    if (__lane_id() == 0) st_status(&st[tile], (tile == first ? kFlagIncl : kFlagAgg) | agg);
+}
+
 // Decoupled look-back for a plain sum (one wave calls it).  Tiles
 // [first, tile) form the scan domain; returns the exclusive prefix of `tile`
-// and publishes its inclusive value.
+// and publishes its inclusive value (and, unless `published`, first its
+// aggregate).
 __device__ uint64_t lookback_sum(uint64_t *st, uint64_t tile, uint64_t first, uint64_t agg,
-                                 uint32_t *ctl)
+                                 uint32_t *ctl, bool published = false)
 {
     const int lane = __lane_id();
-    if (tile == first) {
-        if (lane == 0) st_status(&st[tile], kFlagIncl | agg);
-        return 0;
-    }
-    if (lane == 0) st_status(&st[tile], kFlagAgg | agg);
+    if (!published) lookback_publish(st, tile, first, agg);
+    if (tile == first) return 0;
     uint64_t excl = 0;
     int64_t hi = (int64_t)tile - 1;
     while (true) {
@@ -177,6 +183,44 @@ __device__ uint64_t lookback_sum(uint64_t *st, uint64_t tile, uint64_t first, ui
         hi -= 64;
     }
     if (lane == 0) st_status(&st[tile], kFlagIncl | (excl + agg));
+    return excl;
+}
+
+// Decoupled look-back with the whole workgroup: every step reads the status
+// of kWG predecessors at once (all threads call it; s_lb is 8 u64 of LDS).
+// Returns the exclusive prefix of `tile` in every thread and publishes its
+// inclusive value (and, unless `published`, first its aggregate).
+__device__ uint64_t lookback_sum_wg(uint64_t *st, uint64_t tile, uint64_t first, uint64_t agg,
+                                    uint32_t *ctl, bool published, uint64_t *s_lb)
+{
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    if (!published && tid < 64) lookback_publish(st, tile, first, agg);
+    if (tile == first) return 0;
+    uint64_t excl = 0;
+    int64_t hi = (int64_t)tile - 1;
+    for (uint32_t step = 0;; ++step) {
+        const int64_t idx = hi - tid;
+        const uint64_t s = (idx >= (int64_t)first) ? wait_status(&st[idx], ctl) : kFlagIncl;
+        const uint64_t incl = __ballot((s >> 62) == 2);
+        const int f = incl ? __ffsll((long long)incl) - 1 : 64;
+        // This wave's statuses up to (and including) its nearest inclusive one.
+        const uint64_t upto = wave_sum_u64(lane <= f ? (s & kValMask) : 0);
+        uint64_t *slot = s_lb + (step & 1) * 4;  // alternating: one barrier per step
+        if (lane == 0) slot[wv] = (upto << 1) | (f < 64 ? 1u : 0u);
+        __syncthreads();
+        bool found = false;
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) {
+            const uint64_t e = slot[w];
+            if (!found) {
+                excl += e >> 1;
+                found = e & 1;
+            }
+        }
+        if (found) break;
+        hi -= kWG;
+    }
+    if (tid == 0) st_status(&st[tile], kFlagIncl | (excl + agg));
     return excl;
 }
 
@@ -599,6 +643,124 @@ __device__ __forceinline__ uint32_t round_layout(bool live, bool head, uint32_t 
     return *s_tot;
 }
 
+// Maximum of v over the SPB consecutive lanes of an item (SPB a power of two
+// up to 32; items start at multiples of SPB).  DPP: quad_perm [1,0,3,2] and
+// [2,3,0,1], then the mirror of each 8-lane half-row and of each 16-lane row;
+// the 32-lane step swaps the two rows of each half-wave.
+template <int SPB>
+__device__ __forceinline__ uint32_t item_max(uint32_t v)
+{
+    if (SPB >= 2) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));
+    if (SPB >= 4) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));
+    if (SPB >= 8) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));
+    if (SPB >= 16) v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));
+    if (SPB >= 32) v = max(v, (uint32_t)__shfl_xor((int)v, 16));
+    return v;
+}
+
+// v of lane l - 1 (lane 0 gets `first`): DPP wave_shr:1.
+__device__ __forceinline__ float lane_before(float v, float first)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v),
+                                                      0x138, 0xF, 0xF, false));
+}
+
+// Fast path of k_compress for a simple tile (only full blocks of one chunk)
+// when SPB = B / 8 is a power of two up to 32.  Then K * SPB = 256: round r
+// is the 2048 elements [r * 2048, (r + 1) * 2048) of the tile and thread tid
+// owns elements r * 2048 + 8 * tid + [0, 8).  The whole tile is loaded up
+// front; each block's maximum is reduced across its SPB lanes with DPP; one
+// wave scan per pair of rounds and one barrier give every block its byte
+// offset in the tile; the tile's byte total is published for the look-back
+// before any byte is encoded.  Rounds that fit in ob are encoded there
+// (nbuf of them); the caller writes them out and re-encodes the rest.
+template <int SPB, bool INDEX>
+__device__ __forceinline__ void compress_simple_fast(
+    const float *__restrict__ x, float inv, uint64_t Et, uint32_t nti, uint8_t *ob,
+    uint32_t *s_wtot, uint32_t *s_rbase, uint32_t *s_ioff, uint64_t *status, uint64_t tile,
+    uint64_t &run, uint32_t &nbuf)
+{
+    constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
+
+    float4 va[kMaxRounds], vb[kMaxRounds];
+    float p0[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        va[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        vb[r] = va[r];
+        p0[r] = 0.f;
+        if (r * K + d < nti) {
+            // Et >= 1 (the tile's first item is a block), so e >= 1.
+            const uint64_t e = Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid;
+            __builtin_memcpy(&va[r], x + e, 16);
+            __builtin_memcpy(&vb[r], x + e + 4, 16);
+            if (lane == 0) p0[r] = x[e - 1];
+        }
+    }
+
+    uint32_t mg[kMaxRounds][8], sg[kMaxRounds], bw[kMaxRounds], sz[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        const bool live = r * K + d < nti;
+        const float pv = lane_before(vb[r].w, p0[r]);
+        const uint32_t omax = item_max<SPB>(octet8_deltas(pv, va[r], vb[r], inv, mg[r], sg[r]));
+        bw[r] = live ? bit_width(omax) : 0u;
+        sz[r] = live ? (bw[r] ? 1u + SPB * (bw[r] + 1u) : 1u) : 0u;
+    }
+    // Exclusive byte offset of each lane's block within its wave's part of
+    // the round: one wave scan per pair of rounds (16-bit halves; a wave's
+    // part of a round is at most 64 / SPB * (1 + 33 * SPB) < 2^16 bytes).
+    uint32_t ex[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; r += 2) {
+        const uint32_t v = o == 0 ? (sz[r] | (sz[r + 1] << 16)) : 0u;
+        const uint32_t incl = wave_incl_scan_u32(v, lane);
+        ex[r] = (incl & 0xFFFFu) - sz[r];
+        ex[r + 1] = (incl >> 16) - sz[r + 1];
+        const uint32_t wt = __builtin_amdgcn_readlane(incl, 63);
+        if (lane == 0) {
+            s_wtot[r * 4 + wv] = wt & 0xFFFFu;
+            s_wtot[(r + 1) * 4 + wv] = wt >> 16;
+        }
+    }
+    __syncthreads();
+    uint32_t off[kMaxRounds], rb[kMaxRounds + 1];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        uint32_t before = 0, rt = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t t = s_wtot[r * 4 + w];
+            before += w < wv ? t : 0u;
+            rt += t;
+        }
+        rb[r] = acc;
+        off[r] = acc + before + ex[r];
+        acc += rt;
+    }
+    rb[kMaxRounds] = acc;
+    run = acc;
+    if (tid < 64) lookback_publish(status, tile, 0, acc);
+    nbuf = 0;
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r)
+        if (r * K < nti && rb[r + 1] <= (uint32_t)kOutCap) nbuf = r + 1;
+    if (tid == 0)
+        for (int r = 0; r <= kMaxRounds; ++r) s_rbase[r] = rb[r];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        const bool live = r * K + d < nti;
+        if (INDEX && live && o == 0) s_ioff[r * K + d] = off[r];
+        if ((uint32_t)r < nbuf && live) {
+            if (o == 0) ob[off[r]] = (uint8_t)bw[r];
+            if (bw[r]) encode_octet8(ob, off[r], o, B, bw[r], mg[r], sg[r]);
+        }
+    }
+}
+
 // A tile is g.R rounds of K items.  Each thread owns one octet per round and
 // loads it straight from HBM one round ahead of its use.  Rounds are encoded
 // into the LDS buffer ob (kOutCap bytes) as long as they fit; the first round
@@ -612,7 +774,7 @@ __device__ __forceinline__ uint32_t round_layout(bool live, bool head, uint32_t 
 // (idx_pos[global block]) and, per tile, the q of the element before the
 // tile's first element (idx_carry[tile]), so that a decoder needs neither a
 // parse nor a scan across tiles.
-template <bool INDEX>
+template <bool INDEX, int SPB>
 __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
                                                   const float *__restrict__ d_eb, float eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
@@ -620,7 +782,8 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
                                                   uint32_t *__restrict__ ctl,
                                                   uint64_t *__restrict__ d_size,
                                                   uint64_t *__restrict__ idx_pos,
-                                                  uint32_t *__restrict__ idx_carry)
+                                                  uint32_t *__restrict__ idx_carry,
+                                                  TileOrder ord)
 {
     __shared__ __align__(16) uint8_t ob[kOutCap + 64];
     __shared__ uint32_t s_ioff[INDEX ? kWG * kMaxRounds : 1];
@@ -629,11 +792,13 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     __shared__ uint32_t s_tot;
     __shared__ uint32_t s_rbase[kMaxRounds + 1];
     __shared__ uint32_t s_head_t[kWG], s_head_off[kWG];
+    __shared__ uint32_t s_wtot[kMaxRounds * 4];
+    __shared__ uint64_t s_lb[8];
     __shared__ uint32_t s_nheads;
     __shared__ uint64_t s_prefix;
 
     const int tid = threadIdx.x;
-    const uint64_t tile = blockIdx.x;
+    const uint64_t tile = tile_of_block(ord, blockIdx.x);
     s_max[tid] = 0;
     if (tid == 0) s_nheads = 0;
     const float eb = d_eb ? *d_eb : eb_host;
@@ -649,141 +814,101 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     const uint64_t j_first = cj;
     if (INDEX && tid == 0)
         idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
-    uint32_t nit = (uint32_t)min<uint64_t>(g.K, tIN - tI0);
-    Oct cur = octet_of(g, ct, cj, nit, d, o);
-    OctIn cin;
-    octet_load(x, cur, cin);
+    // A simple tile holds only full blocks of one chunk (B % 8 == 0): every
+    // octet has 8 values at a 32-bit offset from the tile's first element.
+    const uint32_t nti_all = (uint32_t)(tIN - tI0);
+    const bool simple = simple_tile(g, ct, cj, nti_all);
+    uint32_t nit;
     __syncthreads();
 
     uint64_t run = 0;   // tile-relative byte offset of the current round
     uint32_t nbuf = 0;  // rounds held in ob
     bool deferring = false;
-    // A simple tile holds only full blocks of one chunk (B % 8 == 0): every
-    // octet has 8 values at a 32-bit offset from the tile's first element.
-    const uint32_t nti_all = (uint32_t)(tIN - tI0);
-    if (simple_tile(g, ct, cj, nti_all)) {
-        // Element indices are 64-bit: Et >= 1, so Et + off - 1 never wraps.
-        const uint64_t Et = item_elem(g, ct, cj);
-        const uint32_t B = g.B;
-        uint32_t off = simple_octet_off(g, 0, d, o);
-        bool live = d < g.K && d < nti_all;
-        float cp = 0.f, np_ = 0.f;
-        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca, na = ca, nb = ca;
-        if (live) {
-            cp = x[Et + off - 1];
-            __builtin_memcpy(&ca, x + Et + off, 16);
-            __builtin_memcpy(&cb, x + Et + off + 4, 16);
-        }
-        for (uint32_t r = 0; r < g.R; ++r) {
-            const uint32_t first = r * g.K;
-            if (first >= nti_all) break;
-            const uint32_t noff = simple_octet_off(g, r + 1, d, o);
-            const bool nlive = d < g.K && first + g.K + d < nti_all;
-            if (nlive) {
-                np_ = x[Et + noff - 1];
-                __builtin_memcpy(&na, x + Et + noff, 16);
-                __builtin_memcpy(&nb, x + Et + noff + 4, 16);
-            }
-            uint32_t mg[8], sgn = 0;
-            if (live) {
-                const uint32_t omax = octet8_deltas(cp, ca, cb, inv, mg, sgn);
-                if (omax) atomicMax(&s_max[d], omax);
-            }
-            uint32_t b;
-            const uint32_t rt = round_layout(live, false, B, d, o, g.K, s_max, s_off, &s_tot, b);
-            const uint32_t ib = live ? s_off[d] : 0;
-            if (tid == 0) s_rbase[r] = (uint32_t)run;
-            if (INDEX && o == 0 && live) s_ioff[first + d] = (uint32_t)run + ib;
-            if (!deferring && run + rt <= (uint64_t)kOutCap) {
-                if (live) {
-                    if (o == 0) ob[run + ib] = (uint8_t)b;
-                    if (b) encode_octet8(ob, run + ib, o, B, b, mg, sgn);
+    bool published = false;
+    if (SPB && simple) {
+        compress_simple_fast<(SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
+                                                     s_wtot, s_rbase, s_ioff, status, tile, run,
+                                                     nbuf);
+        published = true;
+    } else {
+        // General tiles (chunk heads, short blocks): every round's octet is
+        // loaded before the first is used, so the tile pays one memory
+        // latency and publishes its size about as early as a simple tile.
+        Oct oc[kMaxRounds];
+        OctIn oi[kMaxRounds];
+        {
+            uint32_t t = ct;
+            uint64_t j = cj;
+#pragma unroll
+            for (int r = 0; r < kMaxRounds; ++r) {
+                const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
+                oc[r] = Oct{0, 0, 0, 0, false, false};
+                if ((uint32_t)r < g.R && ri0 < tIN) {
+                    const uint32_t nr = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
+                    oc[r] = octet_of(g, t, j, nr, d, o);
+                    octet_load(x, oc[r], oi[r]);
+                    advance(g, t, j, nr);
                 }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kMaxRounds; ++r) {
+            const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
+            if ((uint32_t)r >= g.R || ri0 >= tIN) break;
+            const Oct &cur = oc[r];
+            uint32_t mg[8], sgn;
+            int32_t qh;
+            s_off[tid] = 0;  // the scan reads all kWG entries; items >= K contribute 0
+            const uint32_t omax = octet_deltas(cur, oi[r], inv, mg, sgn, qh);
+            if (omax) atomicMax(&s_max[d], omax);
+            __syncthreads();
+            uint32_t b = 0, len = 0;
+            if (cur.live) {
+                if (cur.head) {
+                    len = 4;
+                } else {
+                    b = bit_width(s_max[d]);
+                    len = block_bytes(b, cur.L);
+                }
+            }
+            if (o == 0 && d < g.K) s_off[d] = len;
+            __syncthreads();
+            s_max[tid] = 0;
+            if (tid < 64) {
+                const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
+                const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
+                const uint32_t sum = a0 + a1 + a2 + a3;
+                const uint32_t incl = wave_incl_scan_u32(sum, tid);
+                const uint32_t ex = incl - sum;
+                s_off[4 * tid] = ex;
+                s_off[4 * tid + 1] = ex + a0;
+                s_off[4 * tid + 2] = ex + a0 + a1;
+                s_off[4 * tid + 3] = ex + a0 + a1 + a2;
+                if (tid == 63) s_tot = incl;
+            }
+            __syncthreads();
+            const uint32_t rt = s_tot;
+            const uint32_t ib = cur.live ? s_off[d] : 0;
+            if (cur.head && o == 0) {
+                const uint32_t k = atomicAdd(&s_nheads, 1u);
+                s_head_t[k] = cur.t;
+                s_head_off[k] = (uint32_t)run + ib;
+            }
+            if (tid == 0) s_rbase[r] = (uint32_t)run;
+            if (INDEX && o == 0 && cur.live) s_ioff[r * g.K + d] = (uint32_t)run + ib;
+            if (!deferring && run + rt <= (uint64_t)kOutCap) {
+                encode_octet(ob, run + ib, cur, o, b, mg, sgn, qh);
                 nbuf = r + 1;
             } else {
                 deferring = true;
             }
             run += rt;
-            off = noff;
-            live = nlive;
-            cp = np_;
-            ca = na;
-            cb = nb;
             __syncthreads();
         }
-    } else
-    for (uint32_t r = 0; r < g.R; ++r) {
-        const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
-        if (ri0 >= tIN) break;
-        nit = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
-        // Next round's octet: geometry and loads issued before this round's work.
-        uint32_t nt = ct;
-        uint64_t nj = cj;
-        advance(g, nt, nj, nit);
-        const uint64_t rn0 = ri0 + nit;
-        Oct nxt{0, 0, 0, 0, false, false};
-        OctIn nin;
-        if (rn0 < tIN) {
-            nxt = octet_of(g, nt, nj, (uint32_t)min<uint64_t>(g.K, tIN - rn0), d, o);
-            octet_load(x, nxt, nin);
-        }
-
-        uint32_t mg[8], sgn;
-        int32_t qh;
-        s_off[tid] = 0;  // the scan reads all kWG entries; items >= K contribute 0
-        const uint32_t omax = octet_deltas(cur, cin, inv, mg, sgn, qh);
-        if (omax) atomicMax(&s_max[d], omax);
-        __syncthreads();
-        uint32_t b = 0, len = 0;
-        if (cur.live) {
-            if (cur.head) {
-                len = 4;
-            } else {
-                b = bit_width(s_max[d]);
-                len = block_bytes(b, cur.L);
-            }
-        }
-        if (o == 0 && d < g.K) s_off[d] = len;
-        __syncthreads();
-        s_max[tid] = 0;
-        if (tid < 64) {
-            const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
-            const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
-            const uint32_t sum = a0 + a1 + a2 + a3;
-            const uint32_t incl = wave_incl_scan_u32(sum, tid);
-            const uint32_t ex = incl - sum;
-            s_off[4 * tid] = ex;
-            s_off[4 * tid + 1] = ex + a0;
-            s_off[4 * tid + 2] = ex + a0 + a1;
-            s_off[4 * tid + 3] = ex + a0 + a1 + a2;
-            if (tid == 63) s_tot = incl;
-        }
-        __syncthreads();
-        const uint32_t rt = s_tot;
-        const uint32_t ib = cur.live ? s_off[d] : 0;
-        if (cur.head && o == 0) {
-            const uint32_t k = atomicAdd(&s_nheads, 1u);
-            s_head_t[k] = cur.t;
-            s_head_off[k] = (uint32_t)run + ib;
-        }
-        if (tid == 0) s_rbase[r] = (uint32_t)run;
-        if (INDEX && o == 0 && cur.live) s_ioff[r * g.K + d] = (uint32_t)run + ib;
-        if (!deferring && run + rt <= (uint64_t)kOutCap) {
-            encode_octet(ob, run + ib, cur, o, b, mg, sgn, qh);
-            nbuf = r + 1;
-        } else {
-            deferring = true;
-        }
-        run += rt;
-        cur = nxt;
-        cin = nin;
-        ct = nt;
-        cj = nj;
-        __syncthreads();
     }
 
-    if (tid < 64) {
-        const uint64_t p = lookback_sum(status, tile, 0, run, ctl);
+    {
+        const uint64_t p = lookback_sum_wg(status, tile, 0, run, ctl, published, s_lb);
         if (tid == 0) s_prefix = p;
     }
     __syncthreads();
@@ -1786,6 +1911,35 @@ uint32_t *workspace_error_word(void *d_work)
 // Status codes shared with the public header (szp_amd_status).
 enum { kOk = 0, kArg = 1, kCap = 2, kHip = 3 };
 
+// k_compress with the fast simple-tile path compiled for SPB = B / 8 when
+// that is a power of two up to 32 (B = 8 ... 256), else without it.
+template <bool INDEX>
+static void launch_k_compress(const Geom &g, hipStream_t stream, const float *d_x, const float *d_eb,
+                              float eb_host, uint8_t *d_out, uint64_t cap, uint64_t *status,
+                              uint32_t *ctl, uint64_t *d_size, uint64_t *ipos, uint32_t *icarry)
+{
+    const dim3 grid((uint32_t)g.ntiles), block(kWG);
+    const uint32_t spb = (g.B & 7) ? 0u : g.B / 8;
+    const TileOrder ord = general_tiles(g);
+    switch (spb) {
+#define SZP_COMPRESS_CASE(S)                                                                     \
+    case S:                                                                                      \
+        SZP_LAUNCH("k_compress", (k_compress<INDEX, S>), grid, block, 0, stream, d_x, d_eb,      \
+                   eb_host, g, d_out, cap, status, ctl, d_size, ipos, icarry, ord);              \
+        break;
+        SZP_COMPRESS_CASE(1)
+        SZP_COMPRESS_CASE(2)
+        SZP_COMPRESS_CASE(4)
+        SZP_COMPRESS_CASE(8)
+        SZP_COMPRESS_CASE(16)
+        SZP_COMPRESS_CASE(32)
+#undef SZP_COMPRESS_CASE
+    default:
+        SZP_LAUNCH("k_compress", (k_compress<INDEX, 0>), grid, block, 0, stream, d_x, d_eb, eb_host,
+                   g, d_out, cap, status, ctl, d_size, ipos, icarry, ord);
+    }
+}
+
 int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, double rel, int B,
                     int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size, void *d_work,
                     uint64_t work_bytes, hipStream_t stream, void *d_index, uint64_t idx_bytes)
@@ -1828,11 +1982,11 @@ int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, d
         uint64_t *ipos;
         uint32_t *icarry;
         index_layout(g, reinterpret_cast<uint8_t *>(d_index), &ipos, &icarry);
-        SZP_LAUNCH("k_compress", k_compress<true>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream,
-                   d_x, d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size, ipos, icarry);
+        launch_k_compress<true>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl, d_size,
+                                ipos, icarry);
     } else {
-        SZP_LAUNCH("k_compress", k_compress<false>, dim3((uint32_t)g.ntiles), dim3(kWG), 0, stream,
-                   d_x, d_eb, eb_host, g, d_out, cap, w.status, w.ctl, d_size, nullptr, nullptr);
+        launch_k_compress<false>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl, d_size,
+                                 nullptr, nullptr);
     }
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }

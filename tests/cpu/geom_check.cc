@@ -85,6 +85,56 @@ static void check_compress_tile(const Geom &g, uint64_t tile, std::vector<uint8_
     }
 }
 
+// Dispatch order of the compress tiles: general tiles first, then the rest
+// ascending; a bijection of [0, ntiles); listed tiles are exactly the tiles
+// that are not simple (when the list is in use).
+static void check_order(const Geom &g)
+{
+    const TileOrder o = general_tiles(g);
+    for (uint32_t k = 0; k + 1 < o.n; ++k) CHECK(o.t[k] < o.t[k + 1], "general tiles not ascending");
+    for (uint32_t k = 0; k < o.n; ++k) CHECK(o.t[k] < g.ntiles, "general tile past the end");
+    const uint64_t per_tile = (uint64_t)g.K * g.R;
+    auto is_simple = [&](uint64_t tile) {
+        const uint64_t tI0 = tile * per_tile;
+        const uint64_t tIN = tI0 + per_tile < g.nitems ? tI0 + per_tile : g.nitems;
+        uint32_t ct;
+        uint64_t cj;
+        locate(g, tI0, ct, cj);
+        return simple_tile(g, ct, cj, (uint32_t)(tIN - tI0));
+    };
+    auto listed = [&](uint64_t tile) {
+        for (uint32_t k = 0; k < o.n; ++k)
+            if (o.t[k] == tile) return true;
+        return false;
+    };
+    const bool in_use = o.n > 0;
+    if (g.ntiles <= (4ull << 20)) {
+        std::vector<uint8_t> seen(g.ntiles, 0);
+        uint64_t prev = 0;
+        for (uint64_t b = 0; b < g.ntiles; ++b) {
+            const uint64_t t = tile_of_block(o, b);
+            CHECK(t < g.ntiles, "block %llu maps past the end", (unsigned long long)b);
+            if (t >= g.ntiles) continue;
+            CHECK(!seen[t], "tile %llu dispatched twice", (unsigned long long)t);
+            seen[t] = 1;
+            if (b > o.n) CHECK(t > prev, "non-general tiles out of order");
+            if (b >= o.n) prev = t;
+            if (in_use) CHECK(listed(t) == !is_simple(t), "tile %llu listed %d simple %d",
+                              (unsigned long long)t, (int)listed(t), (int)is_simple(t));
+        }
+    } else {
+        std::mt19937_64 rng(g.ntiles);
+        for (int k = 0; k < 20000; ++k) {
+            const uint64_t b = k < 200 ? (uint64_t)k : rng() % g.ntiles;
+            const uint64_t t = tile_of_block(o, b);
+            CHECK(t < g.ntiles, "block maps past the end");
+            if (in_use && t < g.ntiles) CHECK(listed(t) == !is_simple(t), "listed vs simple");
+        }
+    }
+    // Without a list, every tile takes blockIdx order.
+    if (!in_use) CHECK(tile_of_block(o, 5) == 5, "identity order");
+}
+
 static void check_geometry(uint64_t n, int B, int T, bool full)
 {
     Geom g;
@@ -96,6 +146,7 @@ static void check_geometry(uint64_t n, int B, int T, bool full)
     CHECK(g.K * (g.B + 0ull) <= 2048 || g.B > 2048, "round holds more than 2048 elements");
     // Worst-case bytes of one round fit the decode round buffer.
     CHECK((uint64_t)g.K * block_bytes(32, g.B) <= (uint64_t)kOutBufBytes, "round bytes > buffer");
+    check_order(g);
     std::vector<uint8_t> cover;
     if (full) cover.assign(n, 0);
     if (full) {
