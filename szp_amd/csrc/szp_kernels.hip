@@ -189,7 +189,9 @@ __device__ uint64_t lookback_sum(uint64_t *st, uint64_t tile, uint64_t first, ui
 // Decoupled look-back with the whole workgroup: every step reads the status
 // of kWG predecessors at once (all threads call it; s_lb is 8 u64 of LDS).
 // Returns the exclusive prefix of `tile` in every thread and publishes its
-// inclusive value (and, unless `published`, first its aggregate).
+// inclusive value (and, unless `published`, first its aggregate).  WRAP32:
+// the values are sums mod 2^32 (Lorenzo carries).
+template <bool WRAP32 = false>
 __device__ uint64_t lookback_sum_wg(uint64_t *st, uint64_t tile, uint64_t first, uint64_t agg,
                                     uint32_t *ctl, bool published, uint64_t *s_lb)
 {
@@ -220,7 +222,8 @@ __device__ uint64_t lookback_sum_wg(uint64_t *st, uint64_t tile, uint64_t first,
         if (found) break;
         hi -= kWG;
     }
-    if (tid == 0) st_status(&st[tile], kFlagIncl | (excl + agg));
+    if (WRAP32) excl = (uint32_t)excl;
+    if (tid == 0) st_status(&st[tile], kFlagIncl | (WRAP32 ? (uint32_t)(excl + agg) : excl + agg));
     return excl;
 }
 
@@ -238,23 +241,11 @@ __device__ void copy_out_bytes(uint8_t *__restrict__ out, uint64_t G0, uint32_t 
     const uint32_t head = (uint32_t)(a0 - G0), tail = (uint32_t)(G1 - a1);
     if ((uint32_t)tid < head) out[G0 + tid] = ob[tid];
     if ((uint32_t)tid < tail) out[a1 + tid] = ob[(uint32_t)(a1 - G0) + tid];
-    const uint32_t *ob32 = reinterpret_cast<const uint32_t *>(ob);
     const uint32_t nv = (uint32_t)((a1 - a0) >> 4);
-    const uint32_t sh = head & 3;
     for (uint32_t v = tid; v < nv; v += kWG) {
-        const uint32_t p = head + 16 * v;
-        const uint32_t w = p >> 2;
-        uint32_t d0 = ob32[w], d1 = ob32[w + 1], d2 = ob32[w + 2], d3 = ob32[w + 3];
+        // One 16-byte LDS read at any byte offset (unaligned LDS access).
         uint4 val;
-        if (sh) {
-            uint32_t d4 = ob32[w + 4];
-            val.x = __builtin_amdgcn_alignbyte(d1, d0, sh);
-            val.y = __builtin_amdgcn_alignbyte(d2, d1, sh);
-            val.z = __builtin_amdgcn_alignbyte(d3, d2, sh);
-            val.w = __builtin_amdgcn_alignbyte(d4, d3, sh);
-        } else {
-            val = make_uint4(d0, d1, d2, d3);
-        }
+        __builtin_memcpy(&val, ob + head + 16 * v, 16);
         *reinterpret_cast<uint4 *>(out + a0 + 16ull * v) = val;
     }
 }
@@ -678,7 +669,7 @@ template <int SPB, bool INDEX>
 __device__ __forceinline__ void compress_simple_fast(
     const float *__restrict__ x, float inv, uint64_t Et, uint32_t nti, uint8_t *ob,
     uint32_t *s_wtot, uint32_t *s_rbase, uint32_t *s_ioff, uint64_t *status, uint64_t tile,
-    uint64_t &run, uint32_t &nbuf)
+    uint64_t &run, uint32_t &nbuf, uint32_t *idx_carry)
 {
     constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
@@ -700,6 +691,8 @@ __device__ __forceinline__ void compress_simple_fast(
         }
     }
 
+    // Side index: the q of the element before the tile (thread 0 loaded it).
+    if (INDEX && tid == 0) idx_carry[tile] = (uint32_t)quantize(p0[0], inv);
     uint32_t mg[kMaxRounds][8], sg[kMaxRounds], bw[kMaxRounds], sz[kMaxRounds];
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
@@ -812,12 +805,11 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     locate(g, tI0, ct, cj);
     const uint32_t t_first = ct;
     const uint64_t j_first = cj;
-    if (INDEX && tid == 0)
-        idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
-    // A simple tile holds only full blocks of one chunk (B % 8 == 0): every
-    // octet has 8 values at a 32-bit offset from the tile's first element.
     const uint32_t nti_all = (uint32_t)(tIN - tI0);
     const bool simple = simple_tile(g, ct, cj, nti_all);
+    // The fast path writes the carry from data it loads anyway.
+    if (INDEX && tid == 0 && !(SPB && simple))
+        idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
     uint32_t nit;
     __syncthreads();
 
@@ -828,7 +820,7 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     if (SPB && simple) {
         compress_simple_fast<(SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
                                                      s_wtot, s_rbase, s_ioff, status, tile, run,
-                                                     nbuf);
+                                                     nbuf, idx_carry);
         published = true;
     } else {
         // General tiles (chunk heads, short blocks): every round's octet is
@@ -1448,6 +1440,117 @@ __device__ __forceinline__ void load_bytes(const uint8_t *__restrict__ st, uint6
     }
 }
 
+// Fast path of k_decode for a simple tile (full blocks of one chunk, SPB =
+// B / 8 a power of two up to 32), the mirror of compress_simple_fast: thread
+// tid decodes elements r * 2048 + 8 * tid + [0, 8) of every round r.  The
+// block positions of the thread's items and the tile's byte range (its first
+// block and the item after it) are loaded together; the tile's bytes go to
+// LDS once; all rounds are unpacked into registers; one wave scan per round
+// and one barrier give every octet its Lorenzo prefix; the tile's carry comes
+// from the side index or from a look-back over the tiles' delta sums; values
+// fl32((float)q * eb) are stored straight to HBM.  Returns false, before any
+// side effect, when the tile's bytes do not fit the LDS buffer or it is the
+// field's last tile; the caller then takes the general path.
+template <int SPB, bool INDEXED>
+__device__ __forceinline__ bool decode_simple_fast(
+    const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
+    const uint64_t *__restrict__ blkpos, const uint32_t *__restrict__ idx_carry,
+    float *__restrict__ out, uint64_t *status, uint32_t *ctl, uint64_t tile, uint64_t tIN,
+    uint32_t ct, uint64_t cj, uint32_t nti, float eb, uint8_t *buf, uint32_t *s_wsum,
+    uint64_t *s_lb, int *s_bad)
+{
+    constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
+    if (tIN >= g.nitems) return false;
+    const uint64_t pb = (uint64_t)ct * g.nb0 + (cj - 1);  // blkpos index of the first block
+    uint64_t pos[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r)
+        pos[r] = (r * K + d < nti) ? blkpos[pb + r * K + d] : 0;
+    const uint64_t P0 = blkpos[pb];
+    uint32_t t1;
+    uint64_t j1;
+    locate(g, tIN, t1, j1);
+    const uint64_t P1 = j1 ? blkpos[(uint64_t)t1 * g.nb0 + (j1 - 1)]
+                           : g.body + read_u64_le4(st + 4 + 8ull * t1);
+    const uint64_t tbase = P0 & ~15ull;
+    if (!(P0 < P1 && P1 <= len && P1 - tbase <= (uint64_t)kDecTileBytes)) return false;
+
+    load_bytes(st, len, P0, P1, buf);
+    if (tid == 0) *s_bad = 0;
+    __syncthreads();
+    const uint32_t lend = (uint32_t)(P1 - tbase);
+    bool bad = false;
+    uint32_t dl[kMaxRounds][8];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dl[r][i] = 0;
+        if (r * K + d < nti) {
+            if (pos[r] < P0 || pos[r] >= P1) {
+                bad = true;
+            } else {
+                uint32_t h;
+                decode_slot(B, false, o, buf, (uint32_t)(pos[r] - tbase), lend, dl[r], h, bad);
+            }
+        }
+    }
+    if (bad) *s_bad = 1;
+    uint32_t ex[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        const uint32_t v = dl[r][7];
+        const uint32_t incl = wave_incl_scan_u32(v, lane);
+        ex[r] = incl - v;
+        const uint32_t wt = __builtin_amdgcn_readlane(incl, 63);
+        if (lane == 0) s_wsum[r * 4 + wv] = wt;
+    }
+    __syncthreads();
+    uint32_t agg = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxRounds * 4; ++k) agg += s_wsum[k];
+    if (*s_bad) {
+        if (!INDEXED && tid == 0) st_status(&status[tile], kFlagIncl);
+        if (tid == 0) raise_error(ctl, kErrStream);
+        return true;
+    }
+    uint32_t carry;
+    if (INDEXED) {
+        carry = idx_carry[tile];
+    } else {
+        // The tile holds no chunk head: its aggregate is a plain delta sum.
+        if (tid < 64) lookback_publish(status, tile, 0, agg);
+        carry = (uint32_t)lookback_sum_wg<true>(status, tile, 0, agg, ctl, true, s_lb);
+    }
+    const uint64_t Et = item_elem(g, ct, cj);
+    uint32_t base = carry;
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        uint32_t before = base;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) before += w < wv ? s_wsum[r * 4 + w] : 0u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) base += s_wsum[r * 4 + w];
+        if (r * K + d < nti) {
+            const uint32_t q0 = before + ex[r];
+            float4 a, c;
+            a.x = (float)(int32_t)(q0 + dl[r][0]) * eb;
+            a.y = (float)(int32_t)(q0 + dl[r][1]) * eb;
+            a.z = (float)(int32_t)(q0 + dl[r][2]) * eb;
+            a.w = (float)(int32_t)(q0 + dl[r][3]) * eb;
+            c.x = (float)(int32_t)(q0 + dl[r][4]) * eb;
+            c.y = (float)(int32_t)(q0 + dl[r][5]) * eb;
+            c.z = (float)(int32_t)(q0 + dl[r][6]) * eb;
+            c.w = (float)(int32_t)(q0 + dl[r][7]) * eb;
+            float *dst = out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid;
+            __builtin_memcpy(dst, &a, 16);
+            __builtin_memcpy(dst + 4, &c, 16);
+        }
+    }
+    return true;
+}
+
 // Decodes tiles of g.R rounds of K items.  The item positions of the tile
 // (chunk heads from the header, blocks from the parse) and, when they fit,
 // all its stream bytes are loaded once.  Pass 1 unpacks every round to get
@@ -1457,13 +1560,14 @@ __device__ __forceinline__ void load_bytes(const uint8_t *__restrict__ st, uint6
 //
 // With INDEXED, blkpos and idx_carry come from the compressor's side index:
 // the carry is known up front, so the look-back and pass 1 are skipped.
-template <bool VEC, bool INDEXED>
+template <bool INDEXED, int SPB>
 __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
                                                 Geom g, const uint64_t *__restrict__ blkpos,
                                                 const uint32_t *__restrict__ idx_carry,
                                                 float *__restrict__ out,
                                                 uint64_t *__restrict__ status,
-                                                uint32_t *__restrict__ ctl)
+                                                uint32_t *__restrict__ ctl, TileOrder ord,
+                                                bool vec)
 {
     __shared__ __align__(16) uint8_t buf[kDecTileBytes + 64];
     __shared__ __align__(16) float fs[2048 + 8];
@@ -1473,9 +1577,11 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     __shared__ uint64_t s_rE[kMaxRounds + 1];  // first element of each round
     __shared__ uint32_t s_carry;
     __shared__ int s_bad;
+    __shared__ uint32_t s_wsum[kMaxRounds * 4];
+    __shared__ uint64_t s_lb[8];
 
     const int tid = threadIdx.x, lane = __lane_id();
-    const uint64_t tile = blockIdx.x;
+    const uint64_t tile = tile_of_block(ord, blockIdx.x);
     if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
         // Successors may wait on this tile: publish before leaving.
         if (!INDEXED && tid == 0) st_status(&status[tile], kFlagIncl);
@@ -1490,6 +1596,11 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     uint32_t t0;
     uint64_t j0;
     locate(g, tI0, t0, j0);
+    if (SPB && simple_tile(g, t0, j0, nti) &&
+        decode_simple_fast<(SPB ? SPB : 1), INDEXED>(st, len, g, blkpos, idx_carry, out, status,
+                                                     ctl, tile, tIN, t0, j0, nti, eb, buf,
+                                                     s_wsum, s_lb, &s_bad))
+        return;
     if (tid <= (int)kMaxRounds) {
         const uint64_t it = min(tI0 + (uint64_t)tid * g.K, tIN);
         uint32_t t = t0;
@@ -1618,7 +1729,7 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
                 }
                 __syncthreads();
                 const uint64_t a0 = (E0 + 3) & ~3ull, a1 = E1 & ~3ull;
-                if (!VEC || a0 >= a1) {
+                if (!vec || a0 >= a1) {
                     for (uint64_t e = E0 + tid; e < E1; e += kWG) out[e] = fs[e - Eb];
                 } else {
                     if (E0 + tid < a0) out[E0 + tid] = fs[E0 + tid - Eb];
@@ -1940,6 +2051,40 @@ static void launch_k_compress(const Geom &g, hipStream_t stream, const float *d_
     }
 }
 
+// k_decode with the fast simple-tile path compiled for SPB = B / 8 when that
+// is a power of two up to 32.  Without the side index, tiles look back over
+// their predecessors' delta sums, so the general tiles go first (as in
+// compress); with it, tiles are independent and run in order.
+template <bool INDEXED>
+static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
+                            const uint8_t *d_stream, uint64_t len, const uint64_t *blkpos,
+                            const uint32_t *carry, float *d_out, uint64_t *status, uint32_t *ctl)
+{
+    const dim3 grid((uint32_t)g.ntiles), block(kWG);
+    const uint32_t spb = (g.B & 7) ? 0u : g.B / 8;
+    TileOrder ord;
+    ord.n = 0;
+    if (!INDEXED) ord = general_tiles(g);
+    const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+    switch (spb) {
+#define SZP_DECODE_CASE(S)                                                                       \
+    case S:                                                                                      \
+        SZP_LAUNCH(name, (k_decode<INDEXED, S>), grid, block, 0, stream, d_stream, len, g,       \
+                   blkpos, carry, d_out, status, ctl, ord, vec);                                 \
+        break;
+        SZP_DECODE_CASE(1)
+        SZP_DECODE_CASE(2)
+        SZP_DECODE_CASE(4)
+        SZP_DECODE_CASE(8)
+        SZP_DECODE_CASE(16)
+        SZP_DECODE_CASE(32)
+#undef SZP_DECODE_CASE
+    default:
+        SZP_LAUNCH(name, (k_decode<INDEXED, 0>), grid, block, 0, stream, d_stream, len, g, blkpos,
+                   carry, d_out, status, ctl, ord, vec);
+    }
+}
+
 int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, double rel, int B,
                     int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size, void *d_work,
                     uint64_t work_bytes, hipStream_t stream, void *d_index, uint64_t idx_bytes)
@@ -2012,13 +2157,8 @@ int launch_decompress(const uint8_t *d_stream, uint64_t len, uint64_t n, int B, 
     SZP_LAUNCH("k_parse", k_parse, dim3((uint32_t)nt1), dim3(kWG), parse_lds_bytes(B), stream,
                d_stream, len, g, parse_subtile_bytes(B), W, w.ctab, w.exit_st, w.cnt_st, w.blkpos,
                w.ctl);
-    const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
-    if (vec)
-        SZP_LAUNCH("k_decode", (k_decode<true, false>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
-                   stream, d_stream, len, g, w.blkpos, nullptr, d_out, w.dec_st, w.ctl);
-    else
-        SZP_LAUNCH("k_decode", (k_decode<false, false>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
-                   stream, d_stream, len, g, w.blkpos, nullptr, d_out, w.dec_st, w.ctl);
+    launch_k_decode<false>("k_decode", g, stream, d_stream, len, w.blkpos, nullptr, d_out,
+                           w.dec_st, w.ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 
@@ -2040,13 +2180,8 @@ int launch_decompress_indexed(const uint8_t *d_stream, uint64_t len, const void 
     uint64_t *ipos;
     uint32_t *icarry;
     index_layout(g, reinterpret_cast<uint8_t *>(const_cast<void *>(d_index)), &ipos, &icarry);
-    const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
-    if (vec)
-        SZP_LAUNCH("k_decode_indexed", (k_decode<true, true>), dim3((uint32_t)g.ntiles), dim3(kWG), 0,
-                   stream, d_stream, len, g, ipos, icarry, d_out, nullptr, ctl);
-    else
-        SZP_LAUNCH("k_decode_indexed", (k_decode<false, true>), dim3((uint32_t)g.ntiles), dim3(kWG),
-                   0, stream, d_stream, len, g, ipos, icarry, d_out, nullptr, ctl);
+    launch_k_decode<true>("k_decode_indexed", g, stream, d_stream, len, ipos, icarry, d_out,
+                          nullptr, ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 
