@@ -163,10 +163,11 @@ struct TileOrder {
     uint32_t t[kMaxGenTiles];  // ascending
 };
 
-// Tiles holding a chunk head or a chunk's short last block (ascending, each
-// once).  n = 0 when there are more than kMaxGenTiles, or when every tile
-// takes the general path anyway (B % 8 != 0).
-inline TileOrder general_tiles(const Geom &g)
+// Tiles holding a chunk head or a chunk's short last block, and with
+// with_last the field's last tile (ascending, each once).  n = 0 when there
+// are more than kMaxGenTiles, or when every tile takes the general path
+// anyway (B % 8 != 0).
+inline TileOrder general_tiles(const Geom &g, bool with_last = false)
 {
     TileOrder o;
     o.n = 0;
@@ -189,6 +190,7 @@ inline TileOrder general_tiles(const Geom &g)
             return o;
         }
     }
+    if (with_last && g.ntiles && !add(g.ntiles - 1)) o.n = 0;
     return o;
 }
 

@@ -1448,9 +1448,10 @@ __device__ __forceinline__ void load_bytes(const uint8_t *__restrict__ st, uint6
 // LDS once; all rounds are unpacked into registers; one wave scan per round
 // and one barrier give every octet its Lorenzo prefix; the tile's carry comes
 // from the side index or from a look-back over the tiles' delta sums; values
-// fl32((float)q * eb) are stored straight to HBM.  Returns false, before any
-// side effect, when the tile's bytes do not fit the LDS buffer or it is the
-// field's last tile; the caller then takes the general path.
+// fl32((float)q * eb) are stored straight to HBM.  Tiles whose bytes exceed
+// the LDS buffer are read one round at a time.  Returns false, before any
+// side effect, for the field's last tile (its end is not in the index); the
+// caller then takes the general path.
 template <int SPB, bool INDEXED>
 __device__ __forceinline__ bool decode_simple_fast(
     const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
@@ -1474,25 +1475,52 @@ __device__ __forceinline__ bool decode_simple_fast(
     locate(g, tIN, t1, j1);
     const uint64_t P1 = j1 ? blkpos[(uint64_t)t1 * g.nb0 + (j1 - 1)]
                            : g.body + read_u64_le4(st + 4 + 8ull * t1);
-    const uint64_t tbase = P0 & ~15ull;
-    if (!(P0 < P1 && P1 <= len && P1 - tbase <= (uint64_t)kDecTileBytes)) return false;
-
-    load_bytes(st, len, P0, P1, buf);
     if (tid == 0) *s_bad = 0;
-    __syncthreads();
-    const uint32_t lend = (uint32_t)(P1 - tbase);
-    bool bad = false;
+    bool bad = !(P0 < P1 && P1 <= len);
     uint32_t dl[kMaxRounds][8];
 #pragma unroll
-    for (int r = 0; r < kMaxRounds; ++r) {
+    for (int r = 0; r < kMaxRounds; ++r)
 #pragma unroll
         for (int i = 0; i < 8; ++i) dl[r][i] = 0;
-        if (r * K + d < nti) {
-            if (pos[r] < P0 || pos[r] >= P1) {
-                bad = true;
-            } else {
+    if (!bad && P1 - (P0 & ~15ull) <= (uint64_t)kDecTileBytes) {
+        // The whole tile's bytes at once.
+        const uint64_t tbase = P0 & ~15ull;
+        load_bytes(st, len, P0, P1, buf);
+        __syncthreads();
+        const uint32_t lend = (uint32_t)(P1 - tbase);
+#pragma unroll
+        for (int r = 0; r < kMaxRounds; ++r) {
+            if (r * K + d < nti) {
                 uint32_t h;
-                decode_slot(B, false, o, buf, (uint32_t)(pos[r] - tbase), lend, dl[r], h, bad);
+                if (pos[r] < P0 || pos[r] >= P1)
+                    bad = true;
+                else
+                    decode_slot(B, false, o, buf, (uint32_t)(pos[r] - tbase), lend, dl[r], h, bad);
+            }
+        }
+    } else if (!bad) {
+        // A tile too large for the buffer (barely compressible data): one
+        // round's bytes at a time (a round holds at most K * block_bytes(32,
+        // B) bytes, well below the buffer).
+#pragma unroll
+        for (int r = 0; r < kMaxRounds; ++r) {
+            if ((uint32_t)r * K >= nti) break;
+            const uint64_t Pr = blkpos[pb + r * K];
+            const uint64_t Pn = (uint32_t)(r + 1) * K < nti ? blkpos[pb + (r + 1) * K] : P1;
+            const uint64_t rbase = Pr & ~15ull;
+            const bool ok = Pr < Pn && Pn <= P1 && Pn - rbase <= (uint64_t)kDecTileBytes;
+            __syncthreads();  // the previous round is done with buf
+            if (ok) load_bytes(st, len, Pr, Pn, buf);
+            __syncthreads();
+            if (!ok) {
+                bad = true;
+            } else if (r * K + d < nti) {
+                uint32_t h;
+                if (pos[r] < Pr || pos[r] >= Pn)
+                    bad = true;
+                else
+                    decode_slot(B, false, o, buf, (uint32_t)(pos[r] - rbase), (uint32_t)(Pn - rbase),
+                                dl[r], h, bad);
             }
         }
     }
@@ -1566,15 +1594,12 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
                                                 const uint32_t *__restrict__ idx_carry,
                                                 float *__restrict__ out,
                                                 uint64_t *__restrict__ status,
-                                                uint32_t *__restrict__ ctl, TileOrder ord,
-                                                bool vec)
+                                                uint32_t *__restrict__ ctl, TileOrder ord)
 {
     __shared__ __align__(16) uint8_t buf[kDecTileBytes + 64];
-    __shared__ __align__(16) float fs[2048 + 8];
     __shared__ uint32_t s_pos[kWG * kMaxRounds];  // item positions - tile base
     __shared__ uint32_t s_wh[kWG / 64], s_wv[kWG / 64];
     __shared__ uint64_t s_base, s_end;
-    __shared__ uint64_t s_rE[kMaxRounds + 1];  // first element of each round
     __shared__ uint32_t s_carry;
     __shared__ int s_bad;
     __shared__ uint32_t s_wsum[kMaxRounds * 4];
@@ -1601,13 +1626,6 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
                                                      ctl, tile, tIN, t0, j0, nti, eb, buf,
                                                      s_wsum, s_lb, &s_bad))
         return;
-    if (tid <= (int)kMaxRounds) {
-        const uint64_t it = min(tI0 + (uint64_t)tid * g.K, tIN);
-        uint32_t t = t0;
-        uint64_t j = j0;
-        advance(g, t, j, (uint32_t)(it - tI0));
-        s_rE[tid] = it >= g.nitems ? g.n : item_elem(g, t, j);
-    }
 
     // Item positions of the whole tile.
     uint64_t mypos[kMaxRounds];
@@ -1699,46 +1717,30 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
             const uint32_t v = m ? (h ? dl[0] : dl[m - 1]) : 0u;
             uint32_t eh, ev, rh, rv;
             seg_scan_round(h, v, s_wh, s_wv, eh, ev, rh, rv);
-            if (pass == 1) {
+            if (pass == 1 && m) {
                 // carry (+) tile prefix of the round (+) slot prefix in the round
                 uint32_t before = th ? tv : carry + tv;
                 before = eh ? ev : before + ev;
-                const uint64_t E0 = s_rE[r], E1 = s_rE[r + 1];
-                const uint64_t Eb = E0 & ~3ull;
-                if (m) {
-                    const uint32_t fo = (uint32_t)(es - Eb) + (h ? 0u : 8 * o);
-                    if (h) {
-                        fs[fo] = (float)(int32_t)dl[0] * eb;
-                    } else if (m == 8) {
-                        float4 a, c;
-                        a.x = (float)(int32_t)(before + dl[0]) * eb;
-                        a.y = (float)(int32_t)(before + dl[1]) * eb;
-                        a.z = (float)(int32_t)(before + dl[2]) * eb;
-                        a.w = (float)(int32_t)(before + dl[3]) * eb;
-                        c.x = (float)(int32_t)(before + dl[4]) * eb;
-                        c.y = (float)(int32_t)(before + dl[5]) * eb;
-                        c.z = (float)(int32_t)(before + dl[6]) * eb;
-                        c.w = (float)(int32_t)(before + dl[7]) * eb;
-                        __builtin_memcpy(&fs[fo], &a, 16);
-                        __builtin_memcpy(&fs[fo + 4], &c, 16);
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < 8; ++i)
-                            if ((uint32_t)i < m) fs[fo + i] = (float)(int32_t)(before + dl[i]) * eb;
-                    }
-                }
-                __syncthreads();
-                const uint64_t a0 = (E0 + 3) & ~3ull, a1 = E1 & ~3ull;
-                if (!vec || a0 >= a1) {
-                    for (uint64_t e = E0 + tid; e < E1; e += kWG) out[e] = fs[e - Eb];
+                float *dst = out + es + (h ? 0u : 8u * o);
+                if (h) {
+                    dst[0] = (float)(int32_t)dl[0] * eb;
+                } else if (m == 8) {
+                    float4 a, c;
+                    a.x = (float)(int32_t)(before + dl[0]) * eb;
+                    a.y = (float)(int32_t)(before + dl[1]) * eb;
+                    a.z = (float)(int32_t)(before + dl[2]) * eb;
+                    a.w = (float)(int32_t)(before + dl[3]) * eb;
+                    c.x = (float)(int32_t)(before + dl[4]) * eb;
+                    c.y = (float)(int32_t)(before + dl[5]) * eb;
+                    c.z = (float)(int32_t)(before + dl[6]) * eb;
+                    c.w = (float)(int32_t)(before + dl[7]) * eb;
+                    __builtin_memcpy(dst, &a, 16);
+                    __builtin_memcpy(dst + 4, &c, 16);
                 } else {
-                    if (E0 + tid < a0) out[E0 + tid] = fs[E0 + tid - Eb];
-                    if (a1 + tid < E1) out[a1 + tid] = fs[a1 + tid - Eb];
-                    for (uint64_t e = a0 + 4ull * tid; e < a1; e += 4ull * kWG)
-                        *reinterpret_cast<float4 *>(out + e) =
-                            *reinterpret_cast<const float4 *>(&fs[e - Eb]);
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        if ((uint32_t)i < m) dst[i] = (float)(int32_t)(before + dl[i]) * eb;
                 }
-                __syncthreads();
             }
             tv = rh ? rv : tv + rv;
             th |= rh;
@@ -2052,9 +2054,7 @@ static void launch_k_compress(const Geom &g, hipStream_t stream, const float *d_
 }
 
 // k_decode with the fast simple-tile path compiled for SPB = B / 8 when that
-// is a power of two up to 32.  Without the side index, tiles look back over
-// their predecessors' delta sums, so the general tiles go first (as in
-// compress); with it, tiles are independent and run in order.
+// is a power of two up to 32.
 template <bool INDEXED>
 static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
                             const uint8_t *d_stream, uint64_t len, const uint64_t *blkpos,
@@ -2062,15 +2062,16 @@ static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
 {
     const dim3 grid((uint32_t)g.ntiles), block(kWG);
     const uint32_t spb = (g.B & 7) ? 0u : g.B / 8;
-    TileOrder ord;
-    ord.n = 0;
-    if (!INDEXED) ord = general_tiles(g);
-    const bool vec = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+    // General tiles and the last tile (whose end the fast path does not
+    // know) go first: they take the slower general path, and successors
+    // look back over them (without the side index) or the kernel would end
+    // on them.
+    const TileOrder ord = general_tiles(g, true);
     switch (spb) {
 #define SZP_DECODE_CASE(S)                                                                       \
     case S:                                                                                      \
         SZP_LAUNCH(name, (k_decode<INDEXED, S>), grid, block, 0, stream, d_stream, len, g,       \
-                   blkpos, carry, d_out, status, ctl, ord, vec);                                 \
+                   blkpos, carry, d_out, status, ctl, ord);                                      \
         break;
         SZP_DECODE_CASE(1)
         SZP_DECODE_CASE(2)
@@ -2081,7 +2082,7 @@ static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
 #undef SZP_DECODE_CASE
     default:
         SZP_LAUNCH(name, (k_decode<INDEXED, 0>), grid, block, 0, stream, d_stream, len, g, blkpos,
-                   carry, d_out, status, ctl, ord, vec);
+                   carry, d_out, status, ctl, ord);
     }
 }
 

@@ -88,9 +88,9 @@ static void check_compress_tile(const Geom &g, uint64_t tile, std::vector<uint8_
 // Dispatch order of the compress tiles: general tiles first, then the rest
 // ascending; a bijection of [0, ntiles); listed tiles are exactly the tiles
 // that are not simple (when the list is in use).
-static void check_order(const Geom &g)
+static void check_order(const Geom &g, bool with_last)
 {
-    const TileOrder o = general_tiles(g);
+    const TileOrder o = general_tiles(g, with_last);
     for (uint32_t k = 0; k + 1 < o.n; ++k) CHECK(o.t[k] < o.t[k + 1], "general tiles not ascending");
     for (uint32_t k = 0; k < o.n; ++k) CHECK(o.t[k] < g.ntiles, "general tile past the end");
     const uint64_t per_tile = (uint64_t)g.K * g.R;
@@ -119,7 +119,8 @@ static void check_order(const Geom &g)
             seen[t] = 1;
             if (b > o.n) CHECK(t > prev, "non-general tiles out of order");
             if (b >= o.n) prev = t;
-            if (in_use) CHECK(listed(t) == !is_simple(t), "tile %llu listed %d simple %d",
+            const bool special = !is_simple(t) || (with_last && t == g.ntiles - 1);
+            if (in_use) CHECK(listed(t) == special, "tile %llu listed %d simple %d",
                               (unsigned long long)t, (int)listed(t), (int)is_simple(t));
         }
     } else {
@@ -128,7 +129,8 @@ static void check_order(const Geom &g)
             const uint64_t b = k < 200 ? (uint64_t)k : rng() % g.ntiles;
             const uint64_t t = tile_of_block(o, b);
             CHECK(t < g.ntiles, "block maps past the end");
-            if (in_use && t < g.ntiles) CHECK(listed(t) == !is_simple(t), "listed vs simple");
+            const bool special = t < g.ntiles && (!is_simple(t) || (with_last && t == g.ntiles - 1));
+            if (in_use && t < g.ntiles) CHECK(listed(t) == special, "listed vs simple");
         }
     }
     // Without a list, every tile takes blockIdx order.
@@ -146,7 +148,8 @@ static void check_geometry(uint64_t n, int B, int T, bool full)
     CHECK(g.K * (g.B + 0ull) <= 2048 || g.B > 2048, "round holds more than 2048 elements");
     // Worst-case bytes of one round fit the decode round buffer.
     CHECK((uint64_t)g.K * block_bytes(32, g.B) <= (uint64_t)kOutBufBytes, "round bytes > buffer");
-    check_order(g);
+    check_order(g, false);
+    check_order(g, true);
     std::vector<uint8_t> cover;
     if (full) cover.assign(n, 0);
     if (full) {
