@@ -306,73 +306,60 @@ namespace szp_amd {
 // the host emulation in tests/cpu/parse_check.cc) ----
 //
 // tb[p] is byte p of the parse tile; [u, v) is one sub-tile; a chain's
-// positions are tile offsets.  spec/dead/pre are bitmaps over the tile.
+// positions are tile offsets.  spec/pre are bitmaps over the tile.
+//
+// The speculative chain of sub-tile [u, v) is the chain of the first
+// candidate c in [u, min(u + W, v)) that leaves the sub-tile (reaches v)
+// without meeting a width byte > 32.  On the GPU the 64 lanes of a wave try
+// 64 candidates at once with parse_chain_leaves and the lowest one that
+// leaves marks its chain with parse_mark_chain; the host emulation tries the
+// candidates in order with the same two functions.
 
-// Speculative chain of sub-tile [u, v): the first candidate c in
-// [u, min(u + W, v)) whose chain reaches v.  Its block starts are set in
-// spec; positions of failed chains are set in dead.  Returns true and the
-// block count and exit (>= v) on success.
-SZP_HD bool parse_spec_walk(const uint8_t *tb, uint32_t *spec, uint32_t *dead, uint32_t u,
-                            uint32_t v, uint32_t W, uint32_t B, uint32_t &cnt, uint32_t &exit)
+// Walks the chain from c; true when it reaches v (exit = its first position
+// >= v), false when it meets a width byte > 32.
+SZP_HD bool parse_chain_leaves(const uint8_t *tb, uint32_t c, uint32_t v, uint32_t B,
+                               uint32_t &exit)
 {
-    const uint32_t cmax = (u + W < v) ? u + W : v;
-    for (uint32_t c = u; c < cmax; ++c) {
-        if (bm_test(dead, c)) continue;
-        uint32_t p = c, n = 0;
-        bool died = false;
-        while (p < v) {
-            if (bm_test(dead, p)) { died = true; break; }
-            const uint32_t b = tb[p];
-            if (b > 32) { died = true; break; }
-            if (b == 0) {
-                const uint32_t k = zero_run(tb, p, v);
-                bm_set_range(spec, p, k);
-                n += k;
-                p += k;
-            } else {
-                bm_set(spec, p);
-                ++n;
-                p += blen(b, B);
-            }
-        }
-        if (!died) {
-            cnt = n;
-            exit = p;
-            return true;
-        }
-        // Move this attempt's marks from spec to dead: every position of a
-        // chain that dies inside the sub-tile leads to the same death.
-        const uint32_t stop = p;
-        uint32_t q = c;
-        while (q < stop) {
-            const uint32_t b = tb[q];
-            if (b == 0) {
-                uint32_t k = zero_run(tb, q, v);
-                if (k > stop - q) k = stop - q;
-                bm_clear_range(spec, q, k);
-                bm_set_range(dead, q, k);
-                q += k;
-            } else {
-                bm_clear(spec, q);
-                bm_set(dead, q);
-                q += blen(b, B);
-            }
-        }
-        if (stop < v) bm_set(dead, stop);
+    uint32_t p = c;
+    while (p < v) {
+        const uint32_t b = tb[p];
+        if (b > 32) return false;
+        p += b ? blen(b, B) : zero_run(tb, p, v);
     }
-    cnt = 0;
-    exit = v;
-    return false;
+    exit = p;
+    return true;
+}
+
+// Marks the block starts of the chain from c (which leaves the sub-tile) in
+// spec; returns their count.
+SZP_HD uint32_t parse_mark_chain(const uint8_t *tb, uint32_t *spec, uint32_t c, uint32_t v,
+                                 uint32_t B)
+{
+    uint32_t p = c, n = 0;
+    while (p < v) {
+        const uint32_t b = tb[p];
+        if (b == 0) {
+            const uint32_t k = zero_run(tb, p, v);
+            bm_set_range(spec, p, k);
+            n += k;
+            p += k;
+        } else {
+            bm_set(spec, p);
+            ++n;
+            p += blen(b, B);
+        }
+    }
+    return n;
 }
 
 // Chain from candidate c: 0 if it joins spec or dies inside [u, v), else the
 // position (>= v) where it leaves.
-SZP_HD uint32_t parse_cand_walk(const uint8_t *tb, const uint32_t *spec, const uint32_t *dead,
-                                uint32_t c, uint32_t v, uint32_t B)
+SZP_HD uint32_t parse_cand_walk(const uint8_t *tb, const uint32_t *spec, uint32_t c, uint32_t v,
+                                uint32_t B)
 {
     uint32_t p = c;
     while (true) {
-        if (bm_test(spec, p) || bm_test(dead, p)) return 0;
+        if (bm_test(spec, p)) return 0;
         const uint32_t b = tb[p];
         if (b > 32) return 0;
         if (b == 0) {

@@ -1062,10 +1062,11 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
 // is split into four sub-tiles of U bytes, one per wave.  The "entry" of a
 // range is the first block starting in it, its "exit" the first block
 // starting at or after its end.
-//  1. Each wave walks one speculative chain through its sub-tile: the first
-//     candidate c in [u, u + W) (W = longest block) whose chain reaches the
-//     sub-tile end; failed chains are marked dead, so the search is linear.
-//     Runs of zero bytes (b = 0 blocks) are stepped over a word at a time.
+//  1. Each wave finds one speculative chain through its sub-tile: its 64
+//     lanes try 64 candidates c in [u, u + W) (W = longest block) at a time,
+//     and the lowest candidate whose chain reaches the sub-tile end wins; its
+//     block starts are marked.  Runs of zero bytes (b = 0 blocks) are
+//     stepped over a word at a time.
 //  2. For the last sub-tile only, every chain starting in [u, u + W) must
 //     join the speculative chain, die on a width byte > 32 or leave at the
 //     same exit: then the tile's exit is known without its entry and is
@@ -1074,8 +1075,9 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
 //     block, or the predecessor's published exit) is walked until it joins
 //     the sub-tile's speculative chain, then the next sub-tile starts at that
 //     chain's exit.
-//  4. Block counts are scanned with a decoupled look-back over the chunk's
-//     tiles and every block's stream position is written to blkpos[].
+//  4. Block counts are scanned with a decoupled look-back (the whole
+//     workgroup) over the chunk's tiles and every block's stream position is
+//     written to blkpos[].
 // A stream whose chains never synchronise is still parsed exactly, serially.
 __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, uint64_t len,
                                                Geom g, uint32_t U, uint32_t W,
@@ -1089,8 +1091,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     const uint32_t S = 4 * U, NW = S / 32;
     uint8_t *buf = dyn;
     uint32_t *spec = reinterpret_cast<uint32_t *>(dyn + ((S + 64 + 15) & ~15u));
-    uint32_t *dead = spec + NW;
-    uint32_t *pre = dead + NW;
+    uint32_t *pre = spec + NW;
     uint16_t *rspec = reinterpret_cast<uint16_t *>(pre + NW);  // spec bits before word, per sub-tile
     uint16_t *rpre = rspec + NW;
     __shared__ uint64_t s_exit[4];
@@ -1098,6 +1099,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     __shared__ int s_merged[4], s_ok[4], s_broken[4];
     __shared__ int s_nonsync;
     __shared__ uint64_t s_prefix, s_lastpos;
+    __shared__ uint64_t s_lb[8];
 
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint64_t gt = blockIdx.x;
@@ -1129,7 +1131,6 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     }
     for (uint32_t k = tid; k < NW; k += kWG) {
         spec[k] = 0;
-        dead[k] = 0;
         pre[k] = 0;
     }
     if (tid == 0) {
@@ -1139,14 +1140,32 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     __syncthreads();
     const uint8_t *tb = buf + sh;  // tile byte p is tb[p]
 
-    // 1. Speculative chain of each sub-tile (lane 0 of wave w).
+    // 1. Speculative chain of each sub-tile (all lanes of wave w).
     const uint32_t u = wv * U, v = min(u + U, span);
-    if (lane == 0) {
+    {
+        bool ok = false;
         uint32_t cnt = 0, ex = v;
-        const bool ok = u < span && parse_spec_walk(tb, spec, dead, u, v, W, B, cnt, ex);
-        s_ok[wv] = ok;
-        s_cnt[wv] = ok ? cnt : 0;
-        s_exit[wv] = s + (ok ? ex : v);
+        const uint32_t cmax = min(u + W, v);
+        for (uint32_t c0 = u; c0 < cmax; c0 += 64) {  // wave-uniform loop
+            const uint32_t c = c0 + lane;
+            uint32_t e = 0;
+            const bool leaves = c < cmax && parse_chain_leaves(tb, c, v, B, e);
+            const uint64_t m = __ballot(leaves);
+            if (m) {
+                const int w = __ffsll((long long)m) - 1;  // the lowest candidate that leaves
+                uint32_t n = 0;
+                if (lane == w) n = parse_mark_chain(tb, spec, c, v, B);
+                cnt = __builtin_amdgcn_readlane(n, w);
+                ex = __builtin_amdgcn_readlane(e, w);
+                ok = true;
+                break;
+            }
+        }
+        if (lane == 0) {
+            s_ok[wv] = ok;
+            s_cnt[wv] = ok ? cnt : 0;
+            s_exit[wv] = s + (ok ? ex : v);
+        }
     }
     __syncthreads();
 
@@ -1159,7 +1178,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
             const uint32_t cmax = min(u + W, v);
             const uint64_t sx = s_exit[lastw];
             for (uint32_t c = u + lane; c < cmax; c += 64) {
-                const uint32_t leave = parse_cand_walk(tb, spec, dead, c, v, B);
+                const uint32_t leave = parse_cand_walk(tb, spec, c, v, B);
                 if (leave && s + leave != sx) s_nonsync = 1;
             }
         }
@@ -1220,8 +1239,8 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     __syncthreads();
     const uint32_t tile_cnt = (uint32_t)s_prefix;
     __syncthreads();
-    if (tid < 64) {
-        const uint64_t pre_cnt = lookback_sum(cnt_st, gt, first, tile_cnt, ctl);
+    {
+        const uint64_t pre_cnt = lookback_sum_wg(cnt_st, gt, first, tile_cnt, ctl, false, s_lb);
         if (tid == 0) s_prefix = pre_cnt;
     }
     // Pre ranks: pre bits before each word, within its sub-tile.
@@ -1949,7 +1968,7 @@ struct DecompressWs {
 uint32_t parse_subtile_bytes(int B)
 {
     const uint32_t W = block_bytes(32, (uint32_t)B);
-    const uint32_t U = W > 8192 ? W : 8192;
+    const uint32_t U = W > 4096 ? W : 4096;
     return (U + 255) & ~255u;
 }
 
@@ -1958,7 +1977,7 @@ uint32_t parse_tile_bytes(int B) { return 4 * parse_subtile_bytes(B); }
 size_t parse_lds_bytes(int B)
 {
     const uint32_t S = parse_tile_bytes(B), NW = S / 32;
-    return ((S + 64 + 15) & ~15u) + 3ull * NW * 4 + 2ull * NW * 2;
+    return ((S + 64 + 15) & ~15u) + 2ull * NW * 4 + 2ull * NW * 2;
 }
 
 uint64_t decompress_ws_layout(const Geom &g, uint64_t len, uint8_t *base, DecompressWs *w)

@@ -1,9 +1,11 @@
 // Host emulation of the block-boundary parse of a stream that carries no
-// index (k_dsetup + k_parse in szp_amd/csrc/szp_kernels.hip).  The per-thread
-// walks (parse_spec_walk, parse_cand_walk, parse_true_walk, parse_rank) are
-// the szp_geom.h functions the kernel calls; this file replays the kernel's
-// glue tile by tile in blockIdx order, with every LDS array allocated at its
-// exact size so that AddressSanitizer catches any access outside it.
+// index (k_dsetup + k_parse in szp_amd/csrc/szp_kernels.hip).  The per-lane
+// walks (parse_chain_leaves, parse_mark_chain, parse_cand_walk,
+// parse_true_walk, parse_rank) are the szp_geom.h functions the kernel calls;
+// this file replays the kernel's glue tile by tile in blockIdx order (a wave's
+// 64 candidates at a time become one candidate at a time, lowest first, which
+// picks the same chain), with every LDS array allocated at its exact size so
+// that AddressSanitizer catches any access outside it.
 //
 // For each stream it compares the recovered block positions with a plain
 // serial walk of the chunks, and the kernel's stream-error verdict with the
@@ -33,7 +35,7 @@ static uint64_t rd64(const uint8_t *p)
 static uint32_t subtile_bytes(uint32_t B)
 {
     const uint32_t W = block_bytes(32, B);
-    const uint32_t U = W > 8192 ? W : 8192;
+    const uint32_t U = W > 4096 ? W : 4096;
     return (U + 255) & ~255u;
 }
 
@@ -152,10 +154,9 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
         const uint64_t s0 = s & ~15ull;
         const uint32_t sh = (uint32_t)(s - s0);
 
-        // LDS, exact sizes: the byte buffer, three bitmaps, two rank arrays.
+        // LDS, exact sizes: the byte buffer, two bitmaps, two rank arrays.
         std::unique_ptr<uint8_t[]> buf(new uint8_t[S + 64]);
-        std::unique_ptr<uint32_t[]> spec(new uint32_t[NW]), dead(new uint32_t[NW]),
-            pre(new uint32_t[NW]);
+        std::unique_ptr<uint32_t[]> spec(new uint32_t[NW]), pre(new uint32_t[NW]);
         std::unique_ptr<uint16_t[]> rspec(new uint16_t[NW]), rpre(new uint16_t[NW]);
         memset(buf.get(), 0xA5, S + 64);  // stale LDS contents
         for (uint64_t a = s0; a < te; a += 16) {
@@ -166,7 +167,7 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
                     if (a + k < len) buf[a - s0 + k] = st[a + k];
             }
         }
-        for (uint32_t k = 0; k < NW; ++k) spec[k] = dead[k] = pre[k] = 0;
+        for (uint32_t k = 0; k < NW; ++k) spec[k] = pre[k] = 0;
         const uint8_t *tb = buf.get() + sh;
 
         uint64_t s_exit[4];
@@ -175,11 +176,20 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
         int s_nonsync = 0;
         uint64_t s_lastpos = ~0ull;
 
-        // 1. speculative chains
+        // 1. speculative chains: the lowest candidate whose chain leaves
         for (uint32_t wv = 0; wv < 4; ++wv) {
             const uint32_t u = wv * U, v = std::min(u + U, span);
             uint32_t cnt = 0, ex = v;
-            const bool ok = u < span && parse_spec_walk(tb, spec.get(), dead.get(), u, v, W, B, cnt, ex);
+            bool ok = false;
+            const uint32_t cmax = std::min(u + W, v);
+            for (uint32_t c = u; c < cmax && !ok; ++c) {
+                uint32_t e = 0;
+                if (parse_chain_leaves(tb, c, v, B, e)) {
+                    cnt = parse_mark_chain(tb, spec.get(), c, v, B);
+                    ex = e;
+                    ok = true;
+                }
+            }
             s_ok[wv] = ok;
             s_cnt[wv] = ok ? cnt : 0;
             s_exit[wv] = s + (ok ? ex : v);
@@ -193,7 +203,7 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
             } else {
                 const uint32_t cmax = std::min(u + W, v);
                 for (uint32_t c = u; c < cmax; ++c) {
-                    const uint32_t leave = parse_cand_walk(tb, spec.get(), dead.get(), c, v, B);
+                    const uint32_t leave = parse_cand_walk(tb, spec.get(), c, v, B);
                     if (leave && s + leave != s_exit[lastw]) s_nonsync = 1;
                 }
             }
