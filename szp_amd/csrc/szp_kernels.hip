@@ -1069,8 +1069,9 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
 //     stepped over a word at a time.
 //  2. For the last sub-tile only, every chain starting in [u, u + W) must
 //     join the speculative chain, die on a width byte > 32 or leave at the
-//     same exit: then the tile's exit is known without its entry and is
-//     published before the predecessor tile finishes.
+//     same exit (all threads check candidates): then the tile's exit is
+//     known without its entry and is published before the predecessor tile
+//     finishes.
 //  3. One thread resolves the true chain: the tile's entry (the chunk's first
 //     block, or the predecessor's published exit) is walked until it joins
 //     the sub-tile's speculative chain, then the next sub-tile starts at that
@@ -1171,14 +1172,15 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
 
     // 2. Does the last sub-tile's exit depend on its entry?
     const uint32_t lastw = span ? (span - 1) / U : 0;
-    if (wv == (int)lastw && span) {
+    if (span) {  // all threads share the last sub-tile's candidates
+        const uint32_t ul = lastw * U, vl = min(ul + U, span);
         if (!s_ok[lastw]) {
-            if (lane == 0) s_nonsync = 1;
+            if (tid == 0) s_nonsync = 1;
         } else {
-            const uint32_t cmax = min(u + W, v);
+            const uint32_t cmax = min(ul + W, vl);
             const uint64_t sx = s_exit[lastw];
-            for (uint32_t c = u + lane; c < cmax; c += 64) {
-                const uint32_t leave = parse_cand_walk(tb, spec, c, v, B);
+            for (uint32_t c = ul + tid; c < cmax; c += kWG) {
+                const uint32_t leave = parse_cand_walk(tb, spec, c, vl, B);
                 if (leave && s + leave != sx) s_nonsync = 1;
             }
         }
