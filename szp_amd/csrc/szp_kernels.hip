@@ -767,31 +767,41 @@ __device__ __forceinline__ void compress_simple_fast(
 // (idx_pos[global block]) and, per tile, the q of the element before the
 // tile's first element (idx_carry[tile]), so that a decoder needs neither a
 // parse nor a scan across tiles.
+// LDS of one compress workgroup.
+template <bool INDEX>
+struct CompressLds {
+    __align__(16) uint8_t ob[kOutCap + 64];
+    uint32_t s_ioff[INDEX ? kWG * kMaxRounds : 1];
+    uint32_t s_max[kWG];
+    uint32_t s_off[kWG];
+    uint32_t s_tot;
+    uint32_t s_rbase[kMaxRounds + 1];
+    uint32_t s_head_t[kWG], s_head_off[kWG];
+    uint32_t s_wtot[kMaxRounds * 4];
+    uint64_t s_lb[8];
+    uint32_t s_nheads;
+    int s_stop;
+    uint64_t s_prefix;
+};
+
+// One compress tile (all threads of the workgroup call it; every exit is
+// uniform, so a persistent workgroup can go on to its next tile).
 template <bool INDEX, int SPB>
-__global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
-                                                  const float *__restrict__ d_eb, float eb_host,
-                                                  Geom g, uint8_t *__restrict__ out, uint64_t cap,
-                                                  uint64_t *__restrict__ status,
-                                                  uint32_t *__restrict__ ctl,
-                                                  uint64_t *__restrict__ d_size,
-                                                  uint64_t *__restrict__ idx_pos,
-                                                  uint32_t *__restrict__ idx_carry,
-                                                  TileOrder ord)
+__device__ void compress_tile(const float *__restrict__ x, const float *__restrict__ d_eb,
+                              float eb_host, const Geom &g, uint8_t *__restrict__ out,
+                              uint64_t cap, uint64_t *__restrict__ status,
+                              uint32_t *__restrict__ ctl, uint64_t *__restrict__ d_size,
+                              uint64_t *__restrict__ idx_pos, uint32_t *__restrict__ idx_carry,
+                              uint64_t tile, CompressLds<INDEX> &L)
 {
-    __shared__ __align__(16) uint8_t ob[kOutCap + 64];
-    __shared__ uint32_t s_ioff[INDEX ? kWG * kMaxRounds : 1];
-    __shared__ uint32_t s_max[kWG];
-    __shared__ uint32_t s_off[kWG];
-    __shared__ uint32_t s_tot;
-    __shared__ uint32_t s_rbase[kMaxRounds + 1];
-    __shared__ uint32_t s_head_t[kWG], s_head_off[kWG];
-    __shared__ uint32_t s_wtot[kMaxRounds * 4];
-    __shared__ uint64_t s_lb[8];
-    __shared__ uint32_t s_nheads;
-    __shared__ uint64_t s_prefix;
+    uint8_t *ob = L.ob;
+    uint32_t *s_ioff = L.s_ioff, *s_max = L.s_max, *s_off = L.s_off, *s_rbase = L.s_rbase;
+    uint32_t *s_head_t = L.s_head_t, *s_head_off = L.s_head_off, *s_wtot = L.s_wtot;
+    uint64_t *s_lb = L.s_lb;
+    uint32_t &s_tot = L.s_tot, &s_nheads = L.s_nheads;
+    uint64_t &s_prefix = L.s_prefix;
 
     const int tid = threadIdx.x;
-    const uint64_t tile = tile_of_block(ord, blockIdx.x);
     s_max[tid] = 0;
     if (tid == 0) s_nheads = 0;
     const float eb = d_eb ? *d_eb : eb_host;
@@ -906,10 +916,12 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
     __syncthreads();
     const uint64_t G0 = g.body + s_prefix;
     if (tIN == g.nitems && tid == 0) *d_size = G0 + run;
-    if (G0 + run > cap || (ld_ctl(&ctl[1]) & kErrTimeout)) {
-        if (tid == 0 && G0 + run > cap) raise_error(ctl, kErrCapacity);
-        return;
+    if (tid == 0) {
+        L.s_stop = G0 + run > cap || (ld_ctl(&ctl[1]) & kErrTimeout);
+        if (G0 + run > cap) raise_error(ctl, kErrCapacity);
     }
+    __syncthreads();
+    if (L.s_stop) return;
     if (INDEX) {
         const uint32_t nti = (uint32_t)(tIN - tI0);
         for (uint32_t k = tid; k < nti; k += kWG) {
@@ -1000,6 +1012,22 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
         }
         copy_out_bytes(out, G0 + s_rbase[r], s_tot, ob, tid);
     }
+}
+
+template <bool INDEX, int SPB>
+__global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
+                                                  const float *__restrict__ d_eb, float eb_host,
+                                                  Geom g, uint8_t *__restrict__ out, uint64_t cap,
+                                                  uint64_t *__restrict__ status,
+                                                  uint32_t *__restrict__ ctl,
+                                                  uint64_t *__restrict__ d_size,
+                                                  uint64_t *__restrict__ idx_pos,
+                                                  uint32_t *__restrict__ idx_carry,
+                                                  TileOrder ord)
+{
+    __shared__ CompressLds<INDEX> L;
+    compress_tile<INDEX, SPB>(x, d_eb, eb_host, g, out, cap, status, ctl, d_size, idx_pos,
+                              idx_carry, tile_of_block(ord, blockIdx.x), L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1609,30 +1637,34 @@ __device__ __forceinline__ bool decode_simple_fast(
 //
 // With INDEXED, blkpos and idx_carry come from the compressor's side index:
 // the carry is known up front, so the look-back and pass 1 are skipped.
+// LDS of one decode workgroup.
+struct DecodeLds {
+    __align__(16) uint8_t buf[kDecTileBytes + 64];
+    uint32_t s_pos[kWG * kMaxRounds];  // item positions - tile base
+    uint32_t s_wh[kWG / 64], s_wv[kWG / 64];
+    uint64_t s_base, s_end;
+    uint32_t s_carry;
+    int s_bad;
+    uint32_t s_wsum[kMaxRounds * 4];
+    uint64_t s_lb[8];
+};
+
+// One decode tile (all threads of the workgroup call it; every exit is
+// uniform).
 template <bool INDEXED, int SPB>
-__global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
-                                                Geom g, const uint64_t *__restrict__ blkpos,
-                                                const uint32_t *__restrict__ idx_carry,
-                                                float *__restrict__ out,
-                                                uint64_t *__restrict__ status,
-                                                uint32_t *__restrict__ ctl, TileOrder ord)
+__device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
+                            const uint64_t *__restrict__ blkpos,
+                            const uint32_t *__restrict__ idx_carry, float *__restrict__ out,
+                            uint64_t *__restrict__ status, uint32_t *__restrict__ ctl,
+                            uint64_t tile, DecodeLds &L)
 {
-    __shared__ __align__(16) uint8_t buf[kDecTileBytes + 64];
-    __shared__ uint32_t s_pos[kWG * kMaxRounds];  // item positions - tile base
-    __shared__ uint32_t s_wh[kWG / 64], s_wv[kWG / 64];
-    __shared__ uint64_t s_base, s_end;
-    __shared__ uint32_t s_carry;
-    __shared__ int s_bad;
-    __shared__ uint32_t s_wsum[kMaxRounds * 4];
-    __shared__ uint64_t s_lb[8];
+    uint8_t *buf = L.buf;
+    uint32_t *s_pos = L.s_pos, *s_wh = L.s_wh, *s_wv = L.s_wv;
+    uint64_t &s_base = L.s_base, &s_end = L.s_end;
+    uint32_t &s_carry = L.s_carry;
+    int &s_bad = L.s_bad;
 
     const int tid = threadIdx.x, lane = __lane_id();
-    const uint64_t tile = tile_of_block(ord, blockIdx.x);
-    if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
-        // Successors may wait on this tile: publish before leaving.
-        if (!INDEXED && tid == 0) st_status(&status[tile], kFlagIncl);
-        return;
-    }
     const uint32_t ebu = ((uint32_t)st[0] << 24) | ((uint32_t)st[1] << 16) |
                          ((uint32_t)st[2] << 8) | (uint32_t)st[3];
     const float eb = __uint_as_float(ebu);
@@ -1645,7 +1677,7 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     if (SPB && simple_tile(g, t0, j0, nti) &&
         decode_simple_fast<(SPB ? SPB : 1), INDEXED>(st, len, g, blkpos, idx_carry, out, status,
                                                      ctl, tile, tIN, t0, j0, nti, eb, buf,
-                                                     s_wsum, s_lb, &s_bad))
+                                                     L.s_wsum, L.s_lb, &s_bad))
         return;
 
     // Item positions of the whole tile.
@@ -1802,6 +1834,24 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
         if (!INDEXED) st_status(&status[tile], kFlagIncl);
         raise_error(ctl, kErrStream);
     }
+}
+
+template <bool INDEXED, int SPB>
+__global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
+                                                Geom g, const uint64_t *__restrict__ blkpos,
+                                                const uint32_t *__restrict__ idx_carry,
+                                                float *__restrict__ out,
+                                                uint64_t *__restrict__ status,
+                                                uint32_t *__restrict__ ctl, TileOrder ord)
+{
+    __shared__ DecodeLds L;
+    const uint64_t tile = tile_of_block(ord, blockIdx.x);
+    if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
+        // Successors may wait on this tile: publish before leaving.
+        if (!INDEXED && threadIdx.x == 0) st_status(&status[tile], kFlagIncl);
+        return;
+    }
+    decode_tile<INDEXED, SPB>(st, len, g, blkpos, idx_carry, out, status, ctl, tile, L);
 }
 
 // ---------------------------------------------------------------------------
