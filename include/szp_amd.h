@@ -79,6 +79,55 @@ int szp_amd_decompress_dev_indexed(const unsigned char *d_stream, uint64_t strea
                                    int blockSize, int nbThreads, float *d_out, void *d_work,
                                    uint64_t workBytes, void *stream);
 
+/* Batched forms (no reference counterpart: the reference compresses one
+ * field per call, SZp/src/szp.cc:23-115).  Many fields of any sizes share
+ * one persistent launch: workgroups take tickets of consecutive tiles from a
+ * device-side queue that lists every tile of every field.  All fields of a
+ * batch use the same block size B and team size T; each field's stream is
+ * byte-identical to a single-field call's.
+ *
+ * Compress: in = float[n] (device), out = stream buffer of out_cap bytes
+ * (16-byte aligned, szp_amd_compress_bound() is always enough), err_mode /
+ * abs_eb / rel as for szp_amd_compress_dev, index = optional side index
+ * (szp_amd_index_size() bytes, 256-byte aligned; all fields or none).
+ * Decompress: in = stream (16-byte aligned) of in_bytes bytes, out =
+ * float[n], index = the side index written when it was compressed
+ * (required).  n >= 1 for every field. */
+typedef struct szp_amd_field {
+    const void *in;
+    void *out;
+    uint64_t n;
+    uint64_t in_bytes;
+    uint64_t out_cap;
+    void *index;
+    int err_mode;
+    double abs_eb;
+    double rel;
+} szp_amd_field;
+
+/* A prepared batch (opaque). */
+typedef struct szp_amd_batch {
+    uint64_t opaque[16];
+} szp_amd_batch;
+
+uint64_t szp_amd_batch_workspace_size(const szp_amd_field *fields, int numFields, int blockSize,
+                                      int nbThreads, int decompress);
+
+/* Builds the batch's tables and copies them into d_work (256-byte aligned,
+ * workspace_size bytes); returns after the copy.  d_sizes: device uint64[F]
+ * receiving the compressed sizes (compress; may be NULL for decompress).
+ * The fields' buffers must stay valid while the batch is launched. */
+int szp_amd_batch_prepare(const szp_amd_field *fields, int numFields, int blockSize,
+                          int nbThreads, int decompress, uint64_t *d_sizes, void *d_work,
+                          uint64_t workBytes, szp_amd_batch *batch);
+
+/* Enqueues the batch on `stream` (kernels only, so a graph may capture it
+ * and replay it).  ticket: tiles a workgroup takes at a time (>= 1); grid:
+ * persistent workgroups (0 = one per resident slot).  Status as for the
+ * single-field calls: szp_amd_check_status(d_work, stream). */
+int szp_amd_batch_launch(const szp_amd_batch *batch, uint32_t ticket, uint32_t grid,
+                         void *stream);
+
 /* Synchronises `stream` and returns the status accumulated in d_work. */
 int szp_amd_check_status(void *d_work, void *stream);
 

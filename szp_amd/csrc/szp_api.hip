@@ -308,6 +308,39 @@ int szp_amd_decompress_dev_indexed(const unsigned char *d_stream, uint64_t strea
                                      nbThreads, d_out, d_work, workBytes, (hipStream_t)stream);
 }
 
+static_assert(sizeof(szp_amd::BatchPlan) <= sizeof(szp_amd_batch), "batch plan size");
+static_assert(sizeof(szp_amd_field) == sizeof(szp_amd::BatchField), "field layout");
+
+uint64_t szp_amd_batch_workspace_size(const szp_amd_field *fields, int numFields, int blockSize,
+                                      int nbThreads, int decompress)
+{
+    return batch_workspace_bytes(reinterpret_cast<const szp_amd::BatchField *>(fields), numFields,
+                                 blockSize, nbThreads, decompress);
+}
+
+int szp_amd_batch_prepare(const szp_amd_field *fields, int numFields, int blockSize,
+                          int nbThreads, int decompress, uint64_t *d_sizes, void *d_work,
+                          uint64_t workBytes, szp_amd_batch *batch)
+{
+    if (!batch) return SZP_AMD_ERR_ARG;
+    szp_amd::BatchPlan plan;
+    memset(&plan, 0, sizeof(plan));
+    const int rc = batch_prepare(reinterpret_cast<const szp_amd::BatchField *>(fields), numFields,
+                                 blockSize, nbThreads, decompress, d_sizes, d_work, workBytes,
+                                 &plan);
+    memset(batch, 0, sizeof(*batch));
+    if (rc == 0) memcpy(batch, &plan, sizeof(plan));
+    return rc;
+}
+
+int szp_amd_batch_launch(const szp_amd_batch *batch, uint32_t ticket, uint32_t grid, void *stream)
+{
+    if (!batch) return SZP_AMD_ERR_ARG;
+    szp_amd::BatchPlan plan;
+    memcpy(&plan, batch, sizeof(plan));
+    return batch_launch(plan, ticket, grid, (hipStream_t)stream);
+}
+
 int szp_amd_check_status(void *d_work, void *stream)
 {
     if (!d_work) return SZP_AMD_ERR_ARG;

@@ -37,6 +37,41 @@ int launch_decompress_indexed(const uint8_t *d_stream, uint64_t stream_bytes, co
 int launch_decompress(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int B, int T,
                       float *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream);
 
+// ---- batched forms: many fields in one persistent launch ----
+// One field of a batch.  Compress: in = float[n], out = stream buffer of
+// out_cap bytes.  Decompress: in = stream of in_bytes bytes, out = float[n],
+// index = the side index written when it was compressed (required).
+struct BatchField {
+    const void *in;
+    void *out;
+    uint64_t n;
+    uint64_t in_bytes;
+    uint64_t out_cap;
+    void *index;
+    int err_mode;
+    double abs_eb;
+    double rel;
+};
+
+// Host copy of a prepared batch: where its tables live in the workspace and
+// the sizes its launches need.
+struct BatchPlan {
+    void *work;
+    uint64_t jobs, qmap, smap, parts, status, nstatus;  // byte offsets / u64 count
+    uint64_t tiles, segs;
+    uint32_t F, B, T, spb, index, decode;
+};
+
+uint64_t batch_workspace_bytes(const BatchField *f, int F, int B, int T, int decode);
+// Builds the batch tables on the host and copies them into d_work (returns
+// after the copy).  d_sizes: device u64[F] for compressed sizes (compress).
+int batch_prepare(const BatchField *f, int F, int B, int T, int decode, uint64_t *d_sizes,
+                  void *d_work, uint64_t work_bytes, BatchPlan *plan);
+// Enqueues the batch's kernels only (capturable in a graph).  ticket: queue
+// entries a workgroup takes at a time; grid: persistent workgroups (0 = one
+// per resident slot).
+int batch_launch(const BatchPlan &plan, uint32_t ticket, uint32_t grid, hipStream_t stream);
+
 // Device address of the workspace's error word (u32).
 uint32_t *workspace_error_word(void *d_work);
 

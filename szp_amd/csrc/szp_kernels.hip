@@ -1942,6 +1942,159 @@ void collect_locked()
 
 }  // namespace prof
 
+// ---------------------------------------------------------------------------
+// Batched forms: the fields of a batch share one persistent launch.  The
+// host lists every tile of every field in a queue (field by field; a field's
+// tiles in ascending order for compress) and workgroups take `ticket`
+// consecutive queue entries at a time from one atomic counter, running them
+// in queue order.  A compress tile waits only for lower tiles of its own
+// field, which sit earlier in the queue: taken by a running workgroup, or
+// earlier in the same ticket.  So there is no residency assumption, and
+// batches on concurrent streams cannot starve each other.
+// ---------------------------------------------------------------------------
+
+// One field of a batch, as the kernels see it (device memory).
+struct BatchJob {
+    Geom g;
+    const float *x;      // compress input
+    float *y;            // decompress output
+    const uint8_t *st;   // decompress input
+    uint8_t *out;        // compress output
+    uint64_t cap;        // compress capacity / decompress stream length
+    uint64_t *d_size;    // compressed size
+    uint64_t *status;    // look-back words (compress)
+    uint64_t *idx_pos;   // side index
+    uint32_t *idx_carry;
+    float eb;            // ABS: the bound; REL: written by k_batch_range_final
+    float rel;
+    uint32_t rel_mode;
+    uint32_t seg0, nseg;  // range segments (REL)
+};
+
+struct QEntry {
+    uint32_t job;
+    uint32_t tile;
+};
+
+constexpr uint64_t kBatchSeg = 65536;  // elements per range segment
+
+// Min/max of one segment of a REL field (index-free, as k_range_partial).
+__global__ __launch_bounds__(kWG) void k_batch_range_partial(const BatchJob *__restrict__ jobs,
+                                                             const uint32_t *__restrict__ smap,
+                                                             RangePart *__restrict__ parts)
+{
+    const uint32_t sg = blockIdx.x;
+    const BatchJob &J = jobs[smap[sg]];
+    const uint64_t n = J.g.n;
+    const uint64_t s0 = (uint64_t)(sg - J.seg0) * kBatchSeg;
+    const uint32_t len = (uint32_t)(min(s0 + kBatchSeg, n) - s0);
+    const float *xs = J.x + s0;
+    float mn = __builtin_nanf(""), mx = __builtin_nanf("");
+    if ((reinterpret_cast<uintptr_t>(xs) & 15) == 0) {
+        const uint32_t full = len / (4u * kWG) * (4u * kWG);
+        for (uint32_t v = 4u * threadIdx.x; v < full; v += 4u * kWG) {
+            const float4 q = *reinterpret_cast<const float4 *>(xs + v);
+            mn = fminf(mn, fminf(fminf(q.x, q.y), fminf(q.z, q.w)));
+            mx = fmaxf(mx, fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w)));
+        }
+        for (uint32_t i = full + threadIdx.x; i < len; i += kWG) {
+            mn = fminf(mn, xs[i]);
+            mx = fmaxf(mx, xs[i]);
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < len; i += kWG) {
+            mn = fminf(mn, xs[i]);
+            mx = fmaxf(mx, xs[i]);
+        }
+    }
+    range_reduce_wg(mn, mx);
+    if (threadIdx.x == 0) parts[sg] = RangePart{mn, mx};
+}
+
+// eb of every REL field from its segments (one workgroup per field).
+__global__ __launch_bounds__(kWG) void k_batch_range_final(BatchJob *__restrict__ jobs,
+                                                           const RangePart *__restrict__ parts)
+{
+    BatchJob &J = jobs[blockIdx.x];
+    if (!J.rel_mode) return;
+    float mn = __builtin_nanf(""), mx = __builtin_nanf("");
+    for (uint32_t k = threadIdx.x; k < J.nseg; k += kWG) {
+        const RangePart p = parts[J.seg0 + k];
+        mn = fminf(mn, p.mn);
+        mx = fmaxf(mx, p.mx);
+    }
+    range_reduce_wg(mn, mx);
+    if (threadIdx.x == 0) {
+        float eb;
+        if (isnan(J.x[0])) {
+            eb = J.x[0];
+        } else {
+            const float range = (mx == mn) ? mx - mx : mx - mn;
+            eb = range * J.rel;
+        }
+        J.eb = eb;
+    }
+}
+
+// Takes the next ticket: the first queue entry of `ticket` consecutive ones.
+__device__ __forceinline__ uint64_t take_ticket(uint64_t *queue, uint32_t ticket, uint64_t *s_t)
+{
+    __syncthreads();  // the previous ticket's tiles are done with LDS and s_t
+    if (threadIdx.x == 0) *s_t = atomicAdd(reinterpret_cast<unsigned long long *>(queue), ticket);
+    __syncthreads();
+    return *s_t;
+}
+
+template <bool INDEX, int SPB>
+__global__ __launch_bounds__(kWG) void k_batch_compress(const BatchJob *__restrict__ jobs,
+                                                        const QEntry *__restrict__ qmap,
+                                                        uint64_t total, uint64_t *queue,
+                                                        uint32_t ticket, uint32_t *ctl)
+{
+    __shared__ CompressLds<INDEX> L;
+    __shared__ uint64_t s_t;
+    for (uint64_t t0 = take_ticket(queue, ticket, &s_t); t0 < total;
+         t0 = take_ticket(queue, ticket, &s_t)) {
+        const uint64_t t1 = min(t0 + ticket, total);
+        for (uint64_t q = t0; q < t1; ++q) {
+            const QEntry e = qmap[q];
+            const BatchJob &J = jobs[e.job];
+            const Geom g = J.g;
+            compress_tile<INDEX, SPB>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl, J.d_size,
+                                      J.idx_pos, J.idx_carry, e.tile, L);
+            __syncthreads();
+        }
+    }
+}
+
+template <int SPB>
+__global__ __launch_bounds__(kWG) void k_batch_decode(const BatchJob *__restrict__ jobs,
+                                                      const QEntry *__restrict__ qmap,
+                                                      uint64_t total, uint64_t *queue,
+                                                      uint32_t ticket, uint32_t *ctl)
+{
+    __shared__ DecodeLds L;
+    __shared__ uint64_t s_t;
+    __shared__ int s_stop;
+    for (uint64_t t0 = take_ticket(queue, ticket, &s_t); t0 < total;
+         t0 = take_ticket(queue, ticket, &s_t)) {
+        const uint64_t t1 = min(t0 + ticket, total);
+        for (uint64_t q = t0; q < t1; ++q) {
+            if (threadIdx.x == 0) s_stop = (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) != 0;
+            __syncthreads();
+            const bool stop = s_stop;
+            __syncthreads();  // every thread has read s_stop before it is written again
+            if (stop) continue;  // uniform: a malformed stream stops the batch's decode
+            const QEntry e = qmap[q];
+            const BatchJob &J = jobs[e.job];
+            const Geom g = J.g;
+            decode_tile<true, SPB>(J.st, J.cap, g, J.idx_pos, J.idx_carry, J.y, nullptr, ctl,
+                                   e.tile, L);
+            __syncthreads();
+        }
+    }
+}
+
 #define SZP_LAUNCH(name, kern, grid, block, lds, stream, ...)                    \
     do {                                                                       \
         prof::Scope _scope(name, stream);                                      \
@@ -2254,6 +2407,227 @@ int launch_decompress_indexed(const uint8_t *d_stream, uint64_t len, const void 
     index_layout(g, reinterpret_cast<uint8_t *>(const_cast<void *>(d_index)), &ipos, &icarry);
     launch_k_decode<true>("k_decode_indexed", g, stream, d_stream, len, ipos, icarry, d_out,
                           nullptr, ctl);
+    return hipGetLastError() == hipSuccess ? kOk : kHip;
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms (host side)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct BatchLayout {
+    uint64_t jobs, qmap, smap, parts, status, nstatus, tiles, segs, total;
+};
+
+// Workspace of a batch: [ctl | queue counter + look-back words | jobs | queue
+// map | segment map | range partials].
+bool batch_layout(const BatchField *f, int F, int B, int T, int decode, BatchLayout *L)
+{
+    if (F < 1 || !f) return false;
+    uint64_t tiles = 0, segs = 0, nstat = 1;
+    for (int k = 0; k < F; ++k) {
+        Geom g;
+        if (f[k].n == 0 || !make_geom(f[k].n, B, T, &g)) return false;
+        tiles += g.ntiles;
+        if (!decode) {
+            nstat += g.ntiles;
+            if (f[k].err_mode == 1) segs += (f[k].n + kBatchSeg - 1) / kBatchSeg;
+        }
+    }
+    if (tiles >= (1ull << 32)) return false;
+    uint64_t off = align_up(64);
+    L->status = off;
+    L->nstatus = nstat;
+    off += align_up(8 * nstat);
+    L->jobs = off;
+    off += align_up(sizeof(BatchJob) * (uint64_t)F);
+    L->qmap = off;
+    off += align_up(sizeof(QEntry) * tiles);
+    L->smap = off;
+    off += align_up(4 * (segs ? segs : 1));
+    L->parts = off;
+    off += align_up(sizeof(RangePart) * (segs ? segs : 1));
+    L->tiles = tiles;
+    L->segs = segs;
+    L->total = off;
+    return true;
+}
+
+template <typename K>
+uint32_t resident_grid(K kernel)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWG, 0) != hipSuccess)
+        return 1024;
+    const int g = cus * (per_cu > 0 ? per_cu : 1);
+    return g > 0 ? (uint32_t)g : 1024u;
+}
+
+}  // namespace
+
+uint64_t batch_workspace_bytes(const BatchField *f, int F, int B, int T, int decode)
+{
+    BatchLayout L;
+    return batch_layout(f, F, B, T, decode, &L) ? L.total : 0;
+}
+
+int batch_prepare(const BatchField *f, int F, int B, int T, int decode, uint64_t *d_sizes,
+                  void *d_work, uint64_t work_bytes, BatchPlan *plan)
+{
+    BatchLayout L;
+    if (!plan || !d_work || (reinterpret_cast<uintptr_t>(d_work) & 255)) return kArg;
+    if (!batch_layout(f, F, B, T, decode, &L)) return kArg;
+    if (L.total > work_bytes) return kCap;
+    if (!decode && !d_sizes) return kArg;
+    const bool with_index = f[0].index != nullptr;
+    uint8_t *base = reinterpret_cast<uint8_t *>(d_work);
+    std::vector<BatchJob> jobs((size_t)F);
+    std::vector<QEntry> qmap;
+    std::vector<uint32_t> smap;
+    qmap.reserve(L.tiles);
+    uint64_t st_off = L.status + 8;  // after the queue counter
+    uint64_t seg = 0;
+    for (int k = 0; k < F; ++k) {
+        const BatchField &fk = f[k];
+        Geom g;
+        make_geom(fk.n, B, T, &g);
+        if (!fk.in || !fk.out) return kArg;
+        if ((fk.index != nullptr) != with_index || (decode && !with_index)) return kArg;
+        if (fk.index && (reinterpret_cast<uintptr_t>(fk.index) & 255)) return kArg;
+        BatchJob &J = jobs[(size_t)k];
+        memset(&J, 0, sizeof(J));
+        J.g = g;
+        if (fk.index) index_layout(g, reinterpret_cast<uint8_t *>(fk.index), &J.idx_pos, &J.idx_carry);
+        if (decode) {
+            if ((reinterpret_cast<uintptr_t>(fk.in) & 15) || fk.in_bytes < g.body) return kArg;
+            J.st = reinterpret_cast<const uint8_t *>(fk.in);
+            J.y = reinterpret_cast<float *>(fk.out);
+            J.cap = fk.in_bytes;
+        } else {
+            if ((reinterpret_cast<uintptr_t>(fk.out) & 15) || fk.out_cap < g.body) return kArg;
+            if (fk.err_mode != 0 && fk.err_mode != 1) return kArg;
+            J.x = reinterpret_cast<const float *>(fk.in);
+            J.out = reinterpret_cast<uint8_t *>(fk.out);
+            J.cap = fk.out_cap;
+            J.d_size = d_sizes + k;
+            J.status = reinterpret_cast<uint64_t *>(base + st_off);
+            st_off += 8 * g.ntiles;
+            J.rel_mode = fk.err_mode == 1;
+            J.eb = (float)fk.abs_eb;  // SZp/src/szp.cc:32
+            J.rel = (float)fk.rel;
+            if (J.rel_mode) {
+                J.seg0 = (uint32_t)seg;
+                J.nseg = (uint32_t)((fk.n + kBatchSeg - 1) / kBatchSeg);
+                for (uint32_t s2 = 0; s2 < J.nseg; ++s2) smap.push_back((uint32_t)k);
+                seg += J.nseg;
+            }
+        }
+        // Queue order of this field.  Compress: ascending, so that the tiles
+        // of one ticket (taken together, run in order) never wait for a
+        // later tile of the same ticket.  Decode (no waits with the side
+        // index): the general tiles and the last tile first.
+        TileOrder ord;
+        ord.n = 0;
+        if (decode) ord = general_tiles(g, true);
+        for (uint64_t b = 0; b < g.ntiles; ++b)
+            qmap.push_back(QEntry{(uint32_t)k, (uint32_t)tile_of_block(ord, b)});
+    }
+    if (hipMemcpy(base + L.jobs, jobs.data(), sizeof(BatchJob) * jobs.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(base + L.qmap, qmap.data(), sizeof(QEntry) * qmap.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        (!smap.empty() && hipMemcpy(base + L.smap, smap.data(), 4 * smap.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess))
+        return kHip;
+    plan->work = d_work;
+    plan->jobs = L.jobs;
+    plan->qmap = L.qmap;
+    plan->smap = L.smap;
+    plan->parts = L.parts;
+    plan->status = L.status;
+    plan->nstatus = L.nstatus;
+    plan->tiles = L.tiles;
+    plan->segs = L.segs;
+    plan->F = (uint32_t)F;
+    plan->B = (uint32_t)B;
+    plan->T = (uint32_t)T;
+    plan->spb = (B & 7) ? 0u : (uint32_t)B / 8;
+    if (plan->spb & (plan->spb - 1) || plan->spb > 32) plan->spb = 0;
+    plan->index = with_index;
+    plan->decode = decode != 0;
+    return kOk;
+}
+
+int batch_launch(const BatchPlan &P, uint32_t ticket, uint32_t grid, hipStream_t stream)
+{
+    if (!P.work || !P.F || ticket == 0) return kArg;
+    uint8_t *base = reinterpret_cast<uint8_t *>(P.work);
+    uint32_t *ctl = reinterpret_cast<uint32_t *>(base);
+    uint64_t *status = reinterpret_cast<uint64_t *>(base + P.status);
+    BatchJob *jobs = reinterpret_cast<BatchJob *>(base + P.jobs);
+    const QEntry *qmap = reinterpret_cast<const QEntry *>(base + P.qmap);
+    const uint32_t cgrid = (uint32_t)std::min<uint64_t>((P.nstatus + kWG - 1) / kWG, 4096);
+    SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, ctl, status, P.nstatus);
+    const dim3 block(kWG);
+    if (P.decode) {
+        switch (P.spb) {
+#define SZP_BATCH_DECODE_CASE(S)                                                                 \
+    case S: {                                                                                    \
+        const uint32_t gr = grid ? grid : resident_grid(k_batch_decode<S>);                      \
+        SZP_LAUNCH("k_batch_decode", k_batch_decode<S>, dim3(gr), block, 0, stream, jobs, qmap,  \
+                   P.tiles, status, ticket, ctl);                                                \
+        break;                                                                                   \
+    }
+            SZP_BATCH_DECODE_CASE(0)
+            SZP_BATCH_DECODE_CASE(1)
+            SZP_BATCH_DECODE_CASE(2)
+            SZP_BATCH_DECODE_CASE(4)
+            SZP_BATCH_DECODE_CASE(8)
+            SZP_BATCH_DECODE_CASE(16)
+            SZP_BATCH_DECODE_CASE(32)
+#undef SZP_BATCH_DECODE_CASE
+        }
+        return hipGetLastError() == hipSuccess ? kOk : kHip;
+    }
+    if (P.segs) {
+        SZP_LAUNCH("k_batch_range_partial", k_batch_range_partial, dim3((uint32_t)P.segs), block, 0,
+                   stream, jobs, reinterpret_cast<const uint32_t *>(base + P.smap),
+                   reinterpret_cast<RangePart *>(base + P.parts));
+        SZP_LAUNCH("k_batch_range_final", k_batch_range_final, dim3(P.F), block, 0, stream, jobs,
+                   reinterpret_cast<const RangePart *>(base + P.parts));
+    }
+#define SZP_BATCH_COMPRESS_CASE(I, S)                                                            \
+    case S: {                                                                                    \
+        const uint32_t gr = grid ? grid : resident_grid(k_batch_compress<I, S>);                 \
+        SZP_LAUNCH("k_batch_compress", (k_batch_compress<I, S>), dim3(gr), block, 0, stream,     \
+                   jobs, qmap, P.tiles, status, ticket, ctl);                                    \
+        break;                                                                                   \
+    }
+    if (P.index) {
+        switch (P.spb) {
+            SZP_BATCH_COMPRESS_CASE(true, 0)
+            SZP_BATCH_COMPRESS_CASE(true, 1)
+            SZP_BATCH_COMPRESS_CASE(true, 2)
+            SZP_BATCH_COMPRESS_CASE(true, 4)
+            SZP_BATCH_COMPRESS_CASE(true, 8)
+            SZP_BATCH_COMPRESS_CASE(true, 16)
+            SZP_BATCH_COMPRESS_CASE(true, 32)
+        }
+    } else {
+        switch (P.spb) {
+            SZP_BATCH_COMPRESS_CASE(false, 0)
+            SZP_BATCH_COMPRESS_CASE(false, 1)
+            SZP_BATCH_COMPRESS_CASE(false, 2)
+            SZP_BATCH_COMPRESS_CASE(false, 4)
+            SZP_BATCH_COMPRESS_CASE(false, 8)
+            SZP_BATCH_COMPRESS_CASE(false, 16)
+            SZP_BATCH_COMPRESS_CASE(false, 32)
+        }
+    }
+#undef SZP_BATCH_COMPRESS_CASE
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 
