@@ -143,6 +143,176 @@ def eb_report(torch, x, dec, stream_head):
                 strict_over_eb=int((err > eb).sum()), within_rounding_bound=bool((err <= tol).all()))
 
 
+def make_field_1d(torch, kind, n, gen):
+    """One field of the uneven-size mix: 'smooth' (1-D k^-3 spectrum, [-1, 1]),
+    'noise' (U(-1, 1)), or 'zeroblocks' (smooth with 20% of the 128-element
+    index-aligned blocks set to 0)."""
+    if kind == "noise":
+        return torch.rand(n, generator=gen, device="cuda") * 2 - 1
+    m = 1 << max(int(n - 1).bit_length(), 4)  # power-of-two FFT length (few distinct plans)
+    w = torch.randn(m, generator=gen, device="cuda")
+    F = torch.fft.rfft(w)
+    k = torch.fft.rfftfreq(m, device="cuda")
+    k[0] = 1.0
+    F *= k.pow(-1.5)
+    F[0] = 0
+    x = torch.fft.irfft(F, n=m)[:n]
+    mn, mx = x.min(), x.max()
+    x = ((x - mn) / (mx - mn) * 2 - 1).contiguous()
+    if kind == "zeroblocks":
+        nblk = (n + 127) // 128
+        z = torch.rand(nblk, generator=gen, device="cuda") < 0.2
+        m = z.repeat_interleave(128)[:n]
+        x = torch.where(m, torch.zeros_like(x), x)
+    return x.contiguous()
+
+
+def make_mix(torch, count, lo_bytes, hi_bytes, seed):
+    """`count` fields with sizes log-uniform in [lo, hi] bytes, kinds cycling
+    smooth / noise / zeroblocks (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    sizes = np.exp(rng.uniform(np.log(lo_bytes), np.log(hi_bytes), count)).astype(np.int64) // 4
+    kinds = ["smooth", "noise", "zeroblocks"]
+    xs, ks = [], []
+    for i, n in enumerate(sizes):
+        kind = kinds[rng.integers(0, 3)]
+        xs.append(make_field_1d(torch, kind, int(max(n, 1)), gen))
+        ks.append(kind)
+    torch.cuda.synchronize()
+    return xs, ks
+
+
+def _pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q))
+
+
+def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
+    """Uneven-size mix: per-field plain calls against one batched launch
+    (autotuned, replayed from a graph), compress and decompress."""
+    from szp_amd import batch as sb
+    xs, kinds = make_mix(torch, count, 64 << 10, 64 << 20, seed)
+    N = sum(x.numel() for x in xs)
+    log("batched leg: %d fields, %.2f GB generated" % (len(xs), 4 * N / 1e9))
+    mode = (szp.REL, 0.0, rel)
+    s = torch.cuda.current_stream()
+
+    # Plain: one single-field call per field, buffers allocated beforehand.
+    codecs = [szp.DeviceCodec(block_size=B, threads=T) for _ in xs]
+    for c, x in zip(codecs, xs):
+        c.compress(x, *mode, index=True)
+    torch.cuda.synchronize()
+    outs = [c._bufs["cout"] for c in codecs]
+    idxs = [c._bufs["cidx"] for c in codecs]
+    sizes = [int(c._bufs["csize"].view(torch.int64)[0].item()) for c in codecs]
+    decs = [torch.empty_like(x) for x in xs]
+    for c, o, sz, x, d, ix in zip(codecs, outs, sizes, xs, decs, idxs):
+        c.decompress(o, sz, x.numel(), out=d, index=ix)
+    torch.cuda.synchronize()
+
+    def plain_compress():
+        ev = []
+        for c, x, o, ix in zip(codecs, xs, outs, idxs):
+            c.compress(x, *mode, out=o, index=ix)
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(s)
+            ev.append(e)
+        return ev
+
+    def plain_decompress():
+        ev = []
+        for c, o, sz, x, d, ix in zip(codecs, outs, sizes, xs, decs, idxs):
+            c.decompress(o, sz, x.numel(), out=d, index=ix)
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(s)
+            ev.append(e)
+        return ev
+
+    def timed_plain(fn):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            ev = fn()
+            ev[-1].synchronize()
+            lat = [e0.elapsed_time(e) for e in ev]
+            if best is None or lat[-1] < best[-1]:
+                best = lat
+        return best
+
+    pc = timed_plain(plain_compress)
+    pd = timed_plain(plain_decompress)
+    log("batched leg: plain calls timed")
+
+    # Batched: one persistent launch for all fields, autotuned, in a graph.
+    cb = sb.CompressBatch(xs, mode, block_size=B, threads=T, index=True)
+    tune_c = sb.autotune(cb)
+    cb.launch()
+    cb.check()
+    bsizes = cb.byte_sizes()
+    same = bsizes == sizes and all(torch.equal(cb.stream(k, sizes[k]), outs[k][:sizes[k]])
+                                   for k in range(len(xs)))
+    gc = cb.graph()
+    db = sb.DecompressBatch([cb.stream(k, bsizes[k]) for k in range(len(xs))], bsizes,
+                            [x.numel() for x in xs], [cb.index_of(k) for k in range(len(xs))],
+                            block_size=B, threads=T)
+    tune_d = sb.autotune(db)
+    db.launch()
+    db.check()
+    same_dec = all(torch.equal(db.outs[k].view(torch.int32), decs[k].view(torch.int32))
+                   for k in range(len(xs)))
+    gd = db.graph()
+
+    def timed_graph(g):
+        ms = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            g.replay()
+            e1.record(s)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return min(ms)
+
+    bc = timed_graph(gc)
+    cb.check()
+    bd = timed_graph(gd)
+    db.check()
+    log("batched leg: batch timed")
+    C = sum(sizes)
+    res = {
+        "workload": "%d float32 fields, sizes log-uniform 64 KB-64 MB (seed %d), kinds smooth k^-3 / "
+                    "U(-1,1) / 20%% zero blocks, REL %g, B=%d, T=%d" % (count, seed, rel, B, T),
+        "fields": count, "elements": N, "input_bytes": 4 * N, "compressed_bytes": C,
+        "kinds": {k: kinds.count(k) for k in set(kinds)},
+        "plain_compress_GBps": round(4 * N / (pc[-1] * 1e-3) / 1e9, 1),
+        "batched_compress_GBps": round(4 * N / (bc * 1e-3) / 1e9, 1),
+        "compress_speedup": round(pc[-1] / bc, 2),
+        "plain_decompress_GBps": round(4 * N / (pd[-1] * 1e-3) / 1e9, 1),
+        "batched_decompress_GBps": round(4 * N / (bd * 1e-3) / 1e9, 1),
+        "decompress_speedup": round(pd[-1] / bd, 2),
+        "plain_compress_latency_ms": {"p50": round(_pct(pc, 50), 3), "p99": round(_pct(pc, 99), 3)},
+        "batched_compress_latency_ms": {"p50": round(bc, 3), "p99": round(bc, 3)},
+        "plain_decompress_latency_ms": {"p50": round(_pct(pd, 50), 3), "p99": round(_pct(pd, 99), 3)},
+        "batched_decompress_latency_ms": {"p50": round(bd, 3), "p99": round(bd, 3)},
+        "latency_note": "time from the start of the sequence to each field's completion; the batch "
+                        "completes all fields together",
+        "autotune": {"compress": {"ticket": tune_c["ticket"], "grid": tune_c["grid"]},
+                     "decompress": {"ticket": tune_d["ticket"], "grid": tune_d["grid"]}},
+        "streams_byte_identical_to_plain": bool(same),
+        "decompress_bit_identical_to_plain": bool(same_dec),
+        "graph": "one hipGraph replay per batch (kernels only)",
+    }
+    del cb, db, gc, gd, codecs, outs, xs
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_calibration(torch, steps):
     """A plain copy of 512 MiB, 1 read + 1 write per byte: its FETCH_SIZE and
     WRITE_SIZE readings per dispatch give the counters' scale."""
@@ -178,6 +348,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (uneven-size) leg")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched leg (JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a plain 512 MiB device-to-device copy (the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -193,6 +365,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(torch, args.steps)
+        return
+    if args.batch_only:
+        log("batched leg")
+        print(json.dumps(batched_bench(torch, szp, T=args.threads)), flush=True)
         return
 
     n3, B, T, rel = args.n, args.block, args.threads, args.rel
@@ -273,6 +449,13 @@ def main():
             variants.append(v)
             del vc, vr
 
+    batched = None
+    if not args.no_batch:
+        log("batched leg: 256 fields")
+        del x
+        torch.cuda.empty_cache()
+        batched = batched_bench(torch, szp, T=T)
+
     line = {
         "metric": "compress/decompress GB/s and %HBM peak, 1 GPU",
         "value": round(value, 2),
@@ -308,6 +491,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 
