@@ -313,6 +313,112 @@ def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
     return res
 
 
+def serving_bench(torch, szp, rate=2000.0, count=400, pool_size=48, B=128, T=16, seed=11,
+                  budget=64 << 30):
+    """Request stream: Poisson arrivals at `rate`/s, 70% compress of a
+    resident field / 30% decompress of a field compressed earlier, payloads
+    log-uniform 16 KB-256 MB from the smooth or noisy generators, REL from
+    {1e-2, 1e-3, 1e-4}, admission bounded by `budget` bytes of HBM."""
+    import numpy as np
+    from szp_amd import serve
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    nbytes = np.exp(rng.uniform(np.log(16 << 10), np.log(256 << 20), pool_size)).astype(np.int64)
+    pool = [make_field_1d(torch, ["smooth", "noise"][rng.integers(0, 2)], int(b // 4), gen)
+            for b in nbytes]
+    ebs = [1e-2, 1e-3, 1e-4]
+    pre = []
+    for x in pool:
+        c = szp.DeviceCodec(block_size=B, threads=T)
+        out, size_t, idx = c.compress(x, szp.REL, 0.0, ebs[rng.integers(0, 3)], index=True)
+        c.check()
+        size = int(size_t.item())
+        pre.append((out[:size].clone(), size, x.numel(), idx.clone()))
+    torch.cuda.synchronize()
+    log("serving leg: pool of %d fields, %.2f GB" % (len(pool), sum(x.numel() for x in pool) * 4 / 1e9))
+
+    def make_requests(k, lam, seed2):
+        r2 = np.random.default_rng(seed2)
+        reqs, t = [], 0.0
+        for rid in range(k):
+            t += r2.exponential(1.0 / lam)
+            i = int(r2.integers(0, len(pool)))
+            if r2.random() < 0.7:
+                q = serve.Request(rid, "compress", t, x=pool[i],
+                                  mode=(szp.REL, 0.0, ebs[int(r2.integers(0, 3))]))
+            else:
+                st, size, n, idx = pre[i]
+                q = serve.Request(rid, "decompress", t, stream=st, nbytes=size, n=n, index=idx)
+            q.result_index = i
+            reqs.append(q)
+        return reqs
+
+    srv = serve.Server(block_size=B, threads=T, budget=budget, streams=3)
+    srv.run(make_requests(32, rate, seed + 1), keep_results=False)  # warm-up
+    # Heavier offered loads (same request mix) show throughput under queueing.
+    loads = []
+    for k, lam in enumerate((rate * 10, rate * 50)):
+        rq = make_requests(count, lam, seed + 10 + k)
+        srv.batches = 0
+        srv.peak = 0
+        t0 = time.perf_counter()
+        sv = srv.run(rq, keep_results=False)
+        w = time.perf_counter() - t0
+        rp = serve.report(sv, w)
+        loads.append({"rate_per_s": lam, "wall_seconds": round(w, 4), "batches": srv.batches,
+                      "compress_latency_ms": {"p50": rp["compress"]["latency_ms_p50"],
+                                              "p99": rp["compress"]["latency_ms_p99"]},
+                      "decompress_latency_ms": {"p50": rp["decompress"]["latency_ms_p50"],
+                                                "p99": rp["decompress"]["latency_ms_p99"]},
+                      "aggregate_GBps": rp["aggregate_GBps"]})
+        del sv, rq
+    reqs = make_requests(count, rate, seed + 2)
+    src = {r.rid: r.result_index for r in reqs}
+    srv.batches = 0
+    srv.peak = 0
+    t0 = time.perf_counter()
+    served = srv.run(reqs, keep_results=True)
+    wall = time.perf_counter() - t0
+    rep = serve.report(served, wall)
+    # Each result against the same request alone (a sample of 40 of each kind).
+    same, checked = True, 0
+    for r in [q for q in served if q.kind == "compress"][:40]:
+        c = szp.DeviceCodec(block_size=B, threads=T)
+        out, size_t = c.compress(r.x, *r.mode)
+        c.check()
+        n = int(size_t.item())
+        same = same and n == r.result_size and torch.equal(out[:n], r.result)
+        checked += 1
+    for r in [q for q in served if q.kind == "decompress"][:40]:
+        st, size, n, idx = pre[src[r.rid]]
+        c = szp.DeviceCodec(block_size=B, threads=T)
+        d = c.decompress(st, size, n, index=idx)
+        c.check()
+        same = same and torch.equal(d.view(torch.int32), r.result.view(torch.int32))
+        checked += 1
+    log("serving leg: %d requests in %d batches" % (len(served), srv.batches))
+    res = {
+        "workload": "%d requests, Poisson arrivals at %g/s (seed %d), 70%% compress / 30%% "
+                    "decompress, payloads log-uniform 16 KB-256 MB (pool of %d resident fields, "
+                    "smooth or noise), REL in {1e-2,1e-3,1e-4}, B=%d, T=%d, budget %d GiB, "
+                    "3 streams" % (count, rate, seed, pool_size, B, T, budget >> 30),
+        "offered_seconds": round(reqs[-1].arrival, 4),
+        "wall_seconds": round(wall, 4),
+        "batches": srv.batches,
+        "peak_admitted_bytes": int(srv.peak),
+        "compress": rep["compress"],
+        "decompress": rep["decompress"],
+        "aggregate_GBps": rep["aggregate_GBps"],
+        "results_same_alone_and_in_mix": bool(same),
+        "results_checked": checked,
+        "heavier_loads": loads,
+    }
+    del srv, served, reqs, pool, pre
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_calibration(torch, steps):
     """A plain copy of 512 MiB, 1 read + 1 write per byte: its FETCH_SIZE and
     WRITE_SIZE readings per dispatch give the counters' scale."""
@@ -349,6 +455,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (uneven-size) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving leg (JSON line)")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving arrivals per second")
     ap.add_argument("--batch-only", action="store_true", help="only the batched leg (JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a plain 512 MiB device-to-device copy (the known-size kernel "
@@ -369,6 +478,10 @@ def main():
     if args.batch_only:
         log("batched leg")
         print(json.dumps(batched_bench(torch, szp, T=args.threads)), flush=True)
+        return
+    if args.serve_only:
+        log("serving leg")
+        print(json.dumps(serving_bench(torch, szp, rate=args.rate, T=args.threads)), flush=True)
         return
 
     n3, B, T, rel = args.n, args.block, args.threads, args.rel
@@ -455,6 +568,10 @@ def main():
         del x
         torch.cuda.empty_cache()
         batched = batched_bench(torch, szp, T=T)
+    served = None
+    if not args.no_serve:
+        log("serving leg")
+        served = serving_bench(torch, szp, rate=args.rate, T=T)
 
     line = {
         "metric": "compress/decompress GB/s and %HBM peak, 1 GPU",
@@ -492,6 +609,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants,
         "batched": batched,
+        "serving": served,
     }
     print(json.dumps(line), flush=True)
 

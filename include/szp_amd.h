@@ -122,9 +122,11 @@ int szp_amd_batch_prepare(const szp_amd_field *fields, int numFields, int blockS
                           uint64_t workBytes, szp_amd_batch *batch);
 
 /* Enqueues the batch on `stream` (kernels only, so a graph may capture it
- * and replay it).  ticket: tiles a workgroup takes at a time (>= 1); grid:
- * persistent workgroups (0 = one per resident slot).  Status as for the
- * single-field calls: szp_amd_check_status(d_work, stream). */
+ * and replay it).  ticket: tiles a workgroup takes at a time (>= 1;
+ * decompress only, compress takes one tile at a time because a tile waits for
+ * the tiles before it to start); grid: persistent workgroups (0 = one per
+ * resident slot).  Status as for the single-field calls:
+ * szp_amd_check_status(d_work, stream). */
 int szp_amd_batch_launch(const szp_amd_batch *batch, uint32_t ticket, uint32_t grid,
                          void *stream);
 

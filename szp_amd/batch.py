@@ -61,7 +61,7 @@ class _Batch:
                                        ctypes.byref(self.plan))
         if rc:
             raise SzpError("szp_amd_batch_prepare", rc)
-        self.ticket, self.grid = 4, 0
+        self.ticket, self.grid = (2 if decode else 1), 0
 
     def launch(self, stream=None, ticket=None, grid=None):
         """Enqueues the batch (kernels only) on `stream`."""
@@ -191,8 +191,11 @@ def autotune(batch, tickets=(1, 2, 4, 8, 16), grid_fractions=(1.0, 0.75, 0.5), r
     """Times the batch under each (ticket, grid) candidate and keeps the
     fastest on the batch (and in a cache under `key`).  grid fractions are of
     the resident grid (CUs x workgroups per CU); the workgroup size is fixed at
-    256 threads by the tile geometry.  Returns {"ticket", "grid", "ms",
-    "trials"}."""
+    256 threads by the tile geometry; compress batches take one tile per
+    ticket, so only the grid is tuned for them.  Returns {"ticket", "grid",
+    "ms", "trials"}."""
+    if not batch.decode:
+        tickets = (1,)
     if key is not None and key in _TUNED:
         best = _TUNED[key]
         batch.ticket, batch.grid = best["ticket"], best["grid"]

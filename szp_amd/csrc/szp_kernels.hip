@@ -2564,6 +2564,11 @@ int batch_prepare(const BatchField *f, int F, int B, int T, int decode, uint64_t
 int batch_launch(const BatchPlan &P, uint32_t ticket, uint32_t grid, hipStream_t stream)
 {
     if (!P.work || !P.F || ticket == 0) return kArg;
+    // A compress tile waits until every lower tile of its field has started;
+    // a workgroup starts the later tiles of a ticket only after the earlier
+    // ones finish, so tickets of several compress tiles would chain the
+    // workgroups one after another.  Compress takes one tile per ticket.
+    if (!P.decode) ticket = 1;
     uint8_t *base = reinterpret_cast<uint8_t *>(P.work);
     uint32_t *ctl = reinterpret_cast<uint32_t *>(base);
     uint64_t *status = reinterpret_cast<uint64_t *>(base + P.status);
