@@ -419,6 +419,93 @@ def serving_bench(torch, szp, rate=2000.0, count=400, pool_size=48, B=128, T=16,
     return res
 
 
+def _blocks(n, B, T):
+    """SZp blocks of a field: each chunk of n_t elements has ceil((n_t - 1) / B)."""
+    tbs = n // T
+    last = n - (T - 1) * tbs
+    nb = lambda m: (m - 1 + B - 1) // B if m else 0
+    return (T - 1) * nb(tbs) + nb(last)
+
+
+def large_field_bench(torch, szp, B=64, T=16, rel=1e-4, seed=5, steps=3, cpu_check=True):
+    """2048 x 2048 x 1024 float32 (2^32 elements, 16 GiB): k^-3 field plus 1%
+    Gaussian noise, REL 1e-4, B=64, compressed and decompressed in one pass
+    each (the prefix stage over ~67 M blocks)."""
+    import numpy as np
+    nx, ny, nz = 2048, 2048, 1024
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    w = torch.randn((nx, ny, nz), generator=g, device="cuda", dtype=torch.float32)
+    F = torch.fft.rfftn(w)
+    del w
+    kx = torch.fft.fftfreq(nx, device="cuda").view(nx, 1, 1)
+    ky = torch.fft.fftfreq(ny, device="cuda").view(1, ny, 1)
+    kz = torch.fft.rfftfreq(nz, device="cuda").view(1, 1, nz // 2 + 1)
+    F *= (kx * kx + ky * ky + kz * kz).clamp_min(1e-12).pow(-0.75)
+    F[0, 0, 0] = 0
+    x = torch.fft.irfftn(F, s=(nx, ny, nz)).view(-1)
+    del F
+    torch.cuda.empty_cache()
+    x = (x - x.mean()) / x.std()
+    x += 0.01 * torch.randn(x.numel(), generator=g, device="cuda")
+    mn, mx = x.min(), x.max()
+    x = ((x - mn) / (mx - mn) * 2 - 1).contiguous()
+    torch.cuda.synchronize()
+    N = x.numel()
+    log("large field: %d elements generated" % N)
+    codec = szp.DeviceCodec(block_size=B, threads=T)
+    out, size_t, idx = codec.compress(x, szp.REL, 0.0, rel, index=True)
+    codec.check()
+    C = int(size_t.item())
+    dec = torch.empty_like(x)
+    codec.decompress(out, C, N, out=dec, index=idx)
+    codec.check()
+    comp_ms = _timed(torch, lambda: codec.compress(x, szp.REL, 0.0, rel, out=out, index=idx), steps)
+    dec_ms = _timed(torch, lambda: codec.decompress(out, C, N, out=dec, index=idx), steps)
+    dec_parse_ms = _timed(torch, lambda: codec.decompress(out, C, N, out=dec), 1)
+    codec.check()
+    log("large field: timed")
+    # eb honoured (rounding-aware bound, as for the 512^3 field), on the GPU in slices.
+    eb = float(np.frombuffer(out[:4].cpu().numpy()[::-1].tobytes(), dtype=np.float32)[0])
+    worst, strict = 0.0, 0
+    ok = True
+    for a in range(0, N, 1 << 28):
+        xa, da = x[a:a + (1 << 28)].double(), dec[a:a + (1 << 28)].double()
+        err = (xa - da).abs()
+        worst = max(worst, float(err.max()))
+        strict += int((err > eb).sum())
+        ok = ok and bool((err <= eb + 2.0 ** -22 * (eb + da.abs())).all())
+    res = {
+        "workload": "2048x2048x1024 float32 (2^32 elements, 16 GiB) k^-3 field + 1%% Gaussian noise "
+                    "in [-1,1] (seed %d), REL %g, B=%d, T=%d" % (seed, rel, B, T),
+        "n_elements": N, "blocks": _blocks(N, B, T), "compressed_bytes": C,
+        "ratio": round(4 * N / C, 4),
+        "compress_GBps": round(4 * N / (comp_ms * 1e-3) / 1e9, 1),
+        "compress_hbm_frac": round((8 * N + C) / (comp_ms * 1e-3) / 8e12, 4),
+        "decompress_GBps": round(4 * N / (dec_ms * 1e-3) / 1e9, 1),
+        "decompress_hbm_frac": round((C + 4 * N) / (dec_ms * 1e-3) / 8e12, 4),
+        "decompress_foreign_stream_GBps": round(4 * N / (dec_parse_ms * 1e-3) / 1e9, 1),
+        "compress_ms": round(comp_ms, 3), "decompress_ms": round(dec_ms, 3),
+        "error_bound": {"eb": eb, "max_err_over_eb": worst / eb, "strict_over_eb": strict,
+                        "within_rounding_bound": ok},
+    }
+    if cpu_check:
+        # The oracle on the host (16 threads) for the same field: byte parity.
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        xh = x.cpu().numpy()
+        t0 = time.perf_counter()
+        want = oracle_lib.compress(xh, szp.REL, 0.0, rel, B, T, workers=16)
+        tc = time.perf_counter() - t0
+        got = out[:C].cpu().numpy()
+        res["stream_byte_identical_to_oracle"] = bool(want.size == C and np.array_equal(want, got))
+        res["oracle_compress_seconds_16_threads"] = round(tc, 2)
+        del xh, want, got
+    del x, out, dec, idx, codec
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_calibration(torch, steps):
     """A plain copy of 512 MiB, 1 read + 1 write per byte: its FETCH_SIZE and
     WRITE_SIZE readings per dispatch give the counters' scale."""
@@ -458,6 +545,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-only", action="store_true", help="only the serving leg (JSON line)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving arrivals per second")
+    ap.add_argument("--no-large", action="store_true", help="skip the 2^32-element field leg")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the 2^32-element field (16 GiB) leg (JSON line)")
     ap.add_argument("--batch-only", action="store_true", help="only the batched leg (JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a plain 512 MiB device-to-device copy (the known-size kernel "
@@ -478,6 +568,10 @@ def main():
     if args.batch_only:
         log("batched leg")
         print(json.dumps(batched_bench(torch, szp, T=args.threads)), flush=True)
+        return
+    if args.large_only:
+        log("large field leg")
+        print(json.dumps(large_field_bench(torch, szp, T=args.threads)), flush=True)
         return
     if args.serve_only:
         log("serving leg")
@@ -572,6 +666,10 @@ def main():
     if not args.no_serve:
         log("serving leg")
         served = serving_bench(torch, szp, rate=args.rate, T=T)
+    large = None
+    if not args.no_large:
+        log("large field leg")
+        large = large_field_bench(torch, szp, T=T, cpu_check=not args.no_cpu)
 
     line = {
         "metric": "compress/decompress GB/s and %HBM peak, 1 GPU",
@@ -610,6 +708,7 @@ def main():
         "variants": variants,
         "batched": batched,
         "serving": served,
+        "large_field": large,
     }
     print(json.dumps(line), flush=True)
 
