@@ -1,12 +1,12 @@
 /*
- * szp.h -- drop-in C ABI for SZp's fastmode1 float32 path, backed by MI355X
- * HIP kernels (libszp_amd.so).
+ * szp.h -- drop-in C ABI for SZp's compress/decompress (fastMode 1 and 2,
+ * float32 and float64), backed by MI355X HIP kernels (libszp_amd.so).
  *
  * Each declaration keeps the reference's name, signature, ownership and
  * error convention; the comment names the reference interface it replaces
- * (paths relative to the SZp repository root).  Only fastMode ==
- * SZP_NONRANDOMACCESS with dataType == SZ_FLOAT is served: other modes return
- * NULL (see szp_amd_last_error()).
+ * (paths relative to the SZp repository root).  fastMode SZP_NONRANDOMACCESS
+ * and SZP_RANDOMACCESS with dataType SZ_FLOAT or SZ_DOUBLE are served; other
+ * modes return NULL (see szp_amd_last_error()).
  *
  * T, the reference's OpenMP team size, is part of the stream format
  * (SZp/src/szp_float.cc:179-181).  These entry points take it from
@@ -50,8 +50,9 @@ unsigned char *szp_compress(int fastMode, int dataType, void *data, size_t *outS
 
 /* Replaces szp_decompress, SZp/include/szp.h:36 (SZp/src/szp.cc:118-179).
  * Like the reference, the stream's extent comes from its own header and
- * blocks, not from byteLength.  Returns a malloc'd float array (caller
- * free()s it) or NULL. */
+ * blocks, not from byteLength, and every fastMode other than 1 decodes as
+ * random access.  Returns a malloc'd float (SZ_FLOAT) or double (SZ_DOUBLE)
+ * array (caller free()s it) or NULL. */
 void *szp_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
                      size_t nbEle, int blockSize);
 
@@ -75,6 +76,77 @@ float *szp_float_decompress_openmp_threadblock(size_t nbEle, float absErrBound, 
  * (SZp/src/szpd_float.cc:320-423). */
 void szp_float_decompress_openmp_threadblock_arg(float *newData, size_t nbEle, float absErrBound,
                                                  int blockSize, unsigned char *cmpBytes);
+
+/* Replaces szp_float_openmp_threadblock_randomaccess, SZp/include/szp_float.h:42-44
+ * (SZp/src/szp_float.cc:752-912): fastMode 2, each chunk cut into units of
+ * blockSize values, a unit = the raw quantized first value and one block of
+ * the others. */
+unsigned char *szp_float_openmp_threadblock_randomaccess(float *oriData, size_t *outSize,
+                                                         float absErrBound, size_t nbEle,
+                                                         int blockSize);
+
+/* Replaces szp_float_openmp_threadblock_randomaccess_arg, SZp/include/szp_float.h:46-48.
+ * `output` must hold szp_amd_compress_bound_typed(SZP_RANDOMACCESS, SZ_FLOAT,
+ * nbEle, blockSize, T) bytes. */
+void szp_float_openmp_threadblock_randomaccess_arg(unsigned char *output, float *oriData,
+                                                   size_t *outSize, float absErrBound,
+                                                   size_t nbEle, int blockSize);
+
+/* Replaces szp_float_decompress_openmp_threadblock_randomaccess,
+ * SZp/include/szpd_float.h:29 (SZp/src/szpd_float.cc:425-530). */
+float *szp_float_decompress_openmp_threadblock_randomaccess(size_t nbEle, float absErrBound,
+                                                            int blockSize, unsigned char *cmpBytes);
+
+/* Replaces szp_float_decompress_openmp_threadblock_randomaccess_arg,
+ * SZp/include/szpd_float.h:31. */
+void szp_float_decompress_openmp_threadblock_randomaccess_arg(float *newData, size_t nbEle,
+                                                              float absErrBound, int blockSize,
+                                                              unsigned char *cmpBytes);
+
+/* Replaces szp_double_openmp_threadblock, SZp/include/szp_double.h:30-31
+ * (SZp/src/szp_double.cc:148-318): the float format with an 8-byte
+ * big-endian eb and the quantizer (int)(x * (1 / eb)) in double. */
+unsigned char *szp_double_openmp_threadblock(double *oriData, size_t *outSize, double absErrBound,
+                                             size_t nbEle, int blockSize);
+
+/* Replaces szp_double_openmp_threadblock_arg, SZp/include/szp_double.h:33-34.
+ * `output` must hold szp_amd_compress_bound_typed(SZ_DOUBLE, nbEle,
+ * blockSize, T) bytes. */
+void szp_double_openmp_threadblock_arg(unsigned char *output, double *oriData, size_t *outSize,
+                                       double absErrBound, size_t nbEle, int blockSize);
+
+/* Replaces szp_double_decompress_openmp_threadblock, SZp/include/szpd_double.h:21
+ * (SZp/src/szpd_double.cc:23-106).  cmpBytes points just after the 8-byte eb. */
+double *szp_double_decompress_openmp_threadblock(size_t nbEle, double absErrBound, int blockSize,
+                                                 unsigned char *cmpBytes);
+
+/* Replaces szp_double_decompress_openmp_threadblock_arg, SZp/include/szpd_double.h:23. */
+void szp_double_decompress_openmp_threadblock_arg(double *newData, size_t nbEle,
+                                                  double absErrBound, int blockSize,
+                                                  unsigned char *cmpBytes);
+
+/* Replaces szp_double_openmp_threadblock_randomaccess, SZp/include/szp_double.h:42-44
+ * (SZp/src/szp_double.cc:759-920). */
+unsigned char *szp_double_openmp_threadblock_randomaccess(double *oriData, size_t *outSize,
+                                                          double absErrBound, size_t nbEle,
+                                                          int blockSize);
+
+/* Replaces szp_double_openmp_threadblock_randomaccess_arg, SZp/include/szp_double.h:46-48. */
+void szp_double_openmp_threadblock_randomaccess_arg(unsigned char *output, double *oriData,
+                                                    size_t *outSize, double absErrBound,
+                                                    size_t nbEle, int blockSize);
+
+/* Replaces szp_double_decompress_openmp_threadblock_randomaccess,
+ * SZp/include/szpd_double.h:29 (SZp/src/szpd_double.cc:431-534). */
+double *szp_double_decompress_openmp_threadblock_randomaccess(size_t nbEle, double absErrBound,
+                                                              int blockSize,
+                                                              unsigned char *cmpBytes);
+
+/* Replaces szp_double_decompress_openmp_threadblock_randomaccess_arg,
+ * SZp/include/szpd_double.h:31. */
+void szp_double_decompress_openmp_threadblock_randomaccess_arg(double *newData, size_t nbEle,
+                                                               double absErrBound, int blockSize,
+                                                               unsigned char *cmpBytes);
 
 #ifdef __cplusplus
 }

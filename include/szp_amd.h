@@ -1,6 +1,6 @@
 /*
- * szp_amd.h -- device-resident API of libszp_amd.so (SZp fastmode1 float32 on
- * MI355X).  Plain pointers and sizes; `stream` is a hipStream_t passed as
+ * szp_amd.h -- device-resident API of libszp_amd.so (SZp compress/decompress
+ * on MI355X: fastMode 1 and 2, float32 and float64).  Plain pointers and sizes; `stream` is a hipStream_t passed as
  * void* (NULL = the default stream).
  *
  * The *_dev entry points enqueue kernels and return at once: no host
@@ -33,7 +33,7 @@ typedef enum szp_amd_status {
     SZP_AMD_ERR_HIP = 3,         /* HIP runtime error */
     SZP_AMD_ERR_STREAM = 4,      /* malformed compressed stream */
     SZP_AMD_ERR_TIMEOUT = 5,     /* a device-side wait exceeded its bound */
-    SZP_AMD_ERR_UNSUPPORTED = 6, /* mode outside fastmode1 float32 */
+    SZP_AMD_ERR_UNSUPPORTED = 6, /* fastMode or dataType the library does not serve */
     SZP_AMD_ERR_NOMEM = 7        /* host or device allocation failed */
 } szp_amd_status;
 
@@ -136,6 +136,45 @@ int szp_amd_check_status(void *d_work, void *stream);
 /* Exact length of a host-resident stream, from its header and block widths. */
 uint64_t szp_amd_stream_length(const unsigned char *bytes, uint64_t nbEle, int blockSize,
                                int nbThreads);
+
+/* Typed forms: fastMode SZP_NONRANDOMACCESS (1) or SZP_RANDOMACCESS (2),
+ * dataType SZ_FLOAT (0) or SZ_DOUBLE (1).  No reference counterpart as
+ * device calls; each stream is byte-identical to the reference's
+ * szp_compress(fastMode, dataType, ...) with T = nbThreads:
+ *   fastMode 1, float:  SZp/src/szp_float.cc:148-309 / SZp/src/szpd_float.cc:23-128
+ *   fastMode 1, double: SZp/src/szp_double.cc:148-318 / SZp/src/szpd_double.cc:23-106
+ *     (an 8-byte big-endian eb; the quantizer (int)(x * (1 / eb)) in double;
+ *     REL bound from the double range, SZp/src/szp.cc:60-84)
+ *   fastMode 2, float:  SZp/src/szp_float.cc:752-870 / SZp/src/szpd_float.cc:425-530
+ *   fastMode 2, double: SZp/src/szp_double.cc:759-920 / SZp/src/szpd_double.cc:431-534
+ *     (each chunk cut into units of blockSize values from its start; a unit
+ *     is the raw quantized first value and one block of the others).
+ * d_data / d_out are float[nbEle] or double[nbEle].  d_index is optional on
+ * both sides: compress writes the side index when given; decompress uses it
+ * when given, else parses the stream.  Workspace and index sizes depend on
+ * fastMode, not on dataType. */
+uint64_t szp_amd_compress_bound_typed(int fastMode, int dataType, uint64_t nbEle, int blockSize,
+                                      int nbThreads);
+uint64_t szp_amd_compress_workspace_size_typed(int fastMode, uint64_t nbEle, int blockSize,
+                                               int nbThreads);
+uint64_t szp_amd_decompress_workspace_size_typed(int fastMode, uint64_t streamBytes,
+                                                 uint64_t nbEle, int blockSize, int nbThreads);
+uint64_t szp_amd_index_size_typed(int fastMode, uint64_t nbEle, int blockSize, int nbThreads);
+
+int szp_amd_compress_dev_typed(int fastMode, int dataType, const void *d_data, uint64_t nbEle,
+                               int errBoundMode, double absErrBound, double relBoundRatio,
+                               int blockSize, int nbThreads, unsigned char *d_out,
+                               uint64_t outCapacity, uint64_t *d_outSize, void *d_index,
+                               uint64_t indexBytes, void *d_work, uint64_t workBytes,
+                               void *stream);
+
+int szp_amd_decompress_dev_typed(int fastMode, int dataType, const unsigned char *d_stream,
+                                 uint64_t streamBytes, const void *d_index, uint64_t indexBytes,
+                                 uint64_t nbEle, int blockSize, int nbThreads, void *d_out,
+                                 void *d_work, uint64_t workBytes, void *stream);
+
+uint64_t szp_amd_stream_length_typed(int fastMode, int dataType, const unsigned char *bytes,
+                                     uint64_t nbEle, int blockSize, int nbThreads);
 
 /* T used by the drop-in entry points of szp.h; set_threads(0) restores auto. */
 int szp_amd_default_threads(void);

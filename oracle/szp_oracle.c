@@ -1,5 +1,6 @@
 /*
- * szp_oracle.c -- CPU restatement of SZp fastmode1 float32 compress/decompress.
+ * szp_oracle.c -- CPU restatement of SZp fastmode1 compress/decompress
+ * (float32, and float64 through the same chunk/block code).
  *
  * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load this library, and only as the
@@ -13,7 +14,11 @@
  *   SZp/src/szp_TypeManager.cc:13-94    byte planes + residual bits (pack)
  *   SZp/src/szp_TypeManager.cc:436-517  (unpack)
  *   SZp/src/szp_TypeManager.cc:820-844,902-944  sign bitmap (pack / unpack)
- *   SZp/include/szp_CompressionToolkit.h:511-527 big-endian eb bytes
+ *   SZp/include/szp_CompressionToolkit.h:511-546 big-endian eb bytes
+ *   SZp/src/szp.cc:60-84, SZp/src/szp_double.cc:148-318,
+ *   SZp/src/szpd_double.cc:23-106       the double path (8-byte eb, FP64 quantizer)
+ *   SZp/src/szp_float.cc:752-870,
+ *   SZp/src/szpd_float.cc:425-530       fastMode 2 (random access) chunks of units
  *
  * Stream format (all multi-byte integers little-endian unless noted):
  *   [eb: f32 big-endian][T x u64 chunk offsets, exclusive scan of chunk sizes]
@@ -84,6 +89,16 @@ float szpo_inverse_bound(float eb)
 int32_t szpo_quantize(float x, float inv)
 {
     double p = (double)x * (double)inv;
+    if (!(p > -2147483649.0 && p < 2147483648.0)) return INT32_MIN;
+    return (int32_t)p;
+}
+
+/* SZp/src/szp_double.cc:185,209,228: inver_bound = 1 / absErrBound in double,
+ * q = (int)(x * inver_bound) with the double product rounded, then the same
+ * x86 truncation rule. */
+int32_t szpo_quantize_f64(double x, double inv)
+{
+    volatile double p = x * inv;
     if (!(p > -2147483649.0 && p < 2147483648.0)) return INT32_MIN;
     return (int32_t)p;
 }
@@ -208,20 +223,35 @@ uint64_t szpo_compress_bound(uint64_t n, int B, int T)
     return total;
 }
 
-/* One chunk of SZp/src/szp_float.cc:201-275 into dst; returns bytes written. */
-static uint64_t compress_chunk(const float *x, uint64_t lo, uint64_t hi, float inv, int B,
-                               uint8_t *dst)
+/* The values of one field: float (f, inv_f) or double (d, inv_d); fast is
+ * the reference's fastMode (1, or 2 for random access). */
+typedef struct {
+    const float *f;
+    const double *d;
+    float inv_f;
+    double inv_d;
+    int fast;
+} Field;
+
+static inline int32_t quant_at(const Field *v, uint64_t i)
+{
+    return v->f ? szpo_quantize(v->f[i], v->inv_f) : szpo_quantize_f64(v->d[i], v->inv_d);
+}
+
+/* One chunk of SZp/src/szp_float.cc:201-275 (SZp/src/szp_double.cc:201-275)
+ * into dst; returns bytes written. */
+static uint64_t compress_chunk(const Field *x, uint64_t lo, uint64_t hi, int B, uint8_t *dst)
 {
     if (lo >= hi) return 0;
     uint8_t *p = dst;
-    int32_t prior = szpo_quantize(x[lo], inv);
+    int32_t prior = quant_at(x, lo);
     memcpy(p, &prior, 4);
     p += 4;
     int32_t *d = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
     for (uint64_t i = lo + 1; i < hi; i += (uint64_t)B) {
         uint64_t L = (i + (uint64_t)B > hi) ? hi - i : (uint64_t)B;
         for (uint64_t j = 0; j < L; ++j) {
-            int32_t cur = szpo_quantize(x[i + j], inv);
+            int32_t cur = quant_at(x, i + j);
             d[j] = (int32_t)((uint32_t)cur - (uint32_t)prior);
             prior = cur;
         }
@@ -229,6 +259,41 @@ static uint64_t compress_chunk(const float *x, uint64_t lo, uint64_t hi, float i
     }
     free(d);
     return (uint64_t)(p - dst);
+}
+
+/* One chunk of fastMode 2 (SZp/src/szp_float.cc:817-870): units of B values
+ * from lo, each [q(first) i32][block of the other len - 1 deltas]; a unit of
+ * one value still writes its zero width byte (:857-862). */
+static uint64_t compress_chunk_ra(const Field *x, uint64_t lo, uint64_t hi, int B, uint8_t *dst)
+{
+    uint8_t *p = dst;
+    int32_t *d = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
+    for (uint64_t i = lo; i < hi; i += (uint64_t)B) {
+        uint64_t len = (i + (uint64_t)B > hi) ? hi - i : (uint64_t)B;
+        int32_t prior = quant_at(x, i);
+        memcpy(p, &prior, 4);
+        p += 4;
+        for (uint64_t j = 1; j < len; ++j) {
+            int32_t cur = quant_at(x, i + j);
+            d[j - 1] = (int32_t)((uint32_t)cur - (uint32_t)prior);
+            prior = cur;
+        }
+        p += szpo_encode_block(d, len - 1, p);
+    }
+    free(d);
+    return (uint64_t)(p - dst);
+}
+
+/* Largest chunk of m values: fastMode 1 = head + blocks of width 32;
+ * fastMode 2 = per unit 4 + 1 + ceil((B-1)/8) bytes plus 4 per other value. */
+static uint64_t chunk_bound(int fast, uint64_t m, int B)
+{
+    if (!m) return 0;
+    if (fast == 2) {
+        uint64_t units = (m + (uint64_t)B - 1) / (uint64_t)B;
+        return units * (5 + ((uint64_t)B - 1 + 7) / 8) + 4 * (m - units);
+    }
+    return 4 + ((m - 1 + B - 1) / B) * (1 + (B + 7) / 8) + 4 * (m - 1);
 }
 
 static void put_be_f32(uint8_t *b, float v)
@@ -246,15 +311,53 @@ static float get_be_f32(const uint8_t *b)
     return v;
 }
 
+static void put_be_f64(uint8_t *b, double v)
+{
+    uint64_t u;
+    memcpy(&u, &v, 8);
+    for (int i = 0; i < 8; ++i) b[i] = (uint8_t)(u >> (56 - 8 * i));
+}
+
+static double get_be_f64(const uint8_t *b)
+{
+    uint64_t u = 0;
+    for (int i = 0; i < 8; ++i) u = (u << 8) | b[i];
+    double v;
+    memcpy(&v, &u, 8);
+    return v;
+}
+
+/* Chunks, header offsets and concatenation (SZp/src/szp_float.cc:175-303);
+ * es = 4 (float) or 8 (double) bytes of eb already written at out[0]. */
+static int compress_body(const Field *x, uint64_t n, uint64_t es, int B, int T, int workers,
+                         uint8_t *out, uint64_t cap, uint64_t *out_size);
+
 int szpo_compress(const float *x, uint64_t n, float eb, int B, int T, int workers,
                   uint8_t *out, uint64_t cap, uint64_t *out_size)
 {
     if (B < 1 || T < 1 || !out || !out_size) return -1;
-    uint64_t hdr = 4 + 8 * (uint64_t)T;
-    if (cap < hdr) return -1;
-    if (workers <= 0) workers = T;
-    float inv = szpo_inverse_bound(eb);
+    if (cap < 4 + 8 * (uint64_t)T) return -1;
+    Field f = {x, NULL, szpo_inverse_bound(eb), 0.0, 1};
     put_be_f32(out, eb);
+    return compress_body(&f, n, 4, B, T, workers, out, cap, out_size);
+}
+
+int szpo_compress_f64(const double *x, uint64_t n, double eb, int B, int T, int workers,
+                      uint8_t *out, uint64_t cap, uint64_t *out_size)
+{
+    if (B < 1 || T < 1 || !out || !out_size) return -1;
+    if (cap < 8 + 8 * (uint64_t)T) return -1;
+    volatile double one = 1.0;
+    Field f = {NULL, x, 0.0f, one / eb, 1};
+    put_be_f64(out, eb);
+    return compress_body(&f, n, 8, B, T, workers, out, cap, out_size);
+}
+
+static int compress_body(const Field *x, uint64_t n, uint64_t es, int B, int T, int workers,
+                         uint8_t *out, uint64_t cap, uint64_t *out_size)
+{
+    uint64_t hdr = es + 8 * (uint64_t)T;
+    if (workers <= 0) workers = T;
 
     uint8_t **part = (uint8_t **)calloc((size_t)T, sizeof(uint8_t *));
     uint64_t *size = (uint64_t *)calloc((size_t)T, sizeof(uint64_t));
@@ -264,14 +367,15 @@ int szpo_compress(const float *x, uint64_t n, float eb, int B, int T, int worker
         uint64_t lo, hi;
         szpo_chunk(n, T, t, &lo, &hi);
         uint64_t m = hi - lo;
-        uint64_t bound = m ? 4 + ((m - 1 + B - 1) / B) * (1 + (B + 7) / 8) + 4 * (m - 1) : 0;
+        uint64_t bound = chunk_bound(x->fast, m, B);
         part[t] = (uint8_t *)malloc(bound ? bound : 1);
         if (!part[t]) { fail = 1; continue; }
-        size[t] = compress_chunk(x, lo, hi, inv, B, part[t]);
+        size[t] = x->fast == 2 ? compress_chunk_ra(x, lo, hi, B, part[t])
+                               : compress_chunk(x, lo, hi, B, part[t]);
     }
     uint64_t off = 0;
     for (int t = 0; t < T; ++t) {
-        memcpy(out + 4 + 8 * (uint64_t)t, &off, 8);
+        memcpy(out + es + 8 * (uint64_t)t, &off, 8);
         off += size[t];
     }
     if (!fail && hdr + off > cap) fail = 1;
@@ -289,17 +393,34 @@ int szpo_compress(const float *x, uint64_t n, float eb, int B, int T, int worker
     return fail ? -1 : 0;
 }
 
+/* x-hat = (V)q * eb in the value type: fl32((float)q * eb) for float
+ * (SZp/src/szpd_float.cc:63-70,101-116), (double)q * eb for double
+ * (SZp/src/szpd_double.cc:45,90). */
+typedef struct {
+    float *f;
+    double *d;
+    float eb_f;
+    double eb_d;
+    int fast;
+} Out;
+
+static inline void put_val(const Out *o, uint64_t i, int32_t q)
+{
+    if (o->f) o->f[i] = (float)q * o->eb_f;
+    else o->d[i] = (double)q * o->eb_d;
+}
+
 /* One chunk of SZp/src/szpd_float.cc:58-117; returns 0 or -1 when the chunk
  * would read past `end`. */
 static int decompress_chunk(const uint8_t *p, const uint8_t *end, uint64_t lo, uint64_t hi,
-                            float eb, int B, float *out)
+                            const Out *out, int B)
 {
     if (lo >= hi) return 0;
     if (p + 4 > end) return -1;
     int32_t prior;
     memcpy(&prior, p, 4);
     p += 4;
-    out[lo] = (float)prior * eb;
+    put_val(out, lo, prior);
     int32_t *d = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
     int rc = 0;
     for (uint64_t i = lo + 1; i < hi; i += (uint64_t)B) {
@@ -308,21 +429,48 @@ static int decompress_chunk(const uint8_t *p, const uint8_t *end, uint64_t lo, u
         p += szpo_decode_block(p, L, d);
         for (uint64_t j = 0; j < L; ++j) {
             prior = (int32_t)((uint32_t)prior + (uint32_t)d[j]);
-            out[i + j] = (float)prior * eb;
+            put_val(out, i + j, prior);
         }
     }
     free(d);
     return rc;
 }
 
-int szpo_decompress(const uint8_t *stream, uint64_t len, uint64_t n, int B, int T, int workers,
-                    float *out)
+/* One chunk of fastMode 2 (SZp/src/szpd_float.cc:470-520): a unit of one
+ * value has no width byte to read (the compressor's zero byte is skipped
+ * only because nothing follows it in the chunk). */
+static int decompress_chunk_ra(const uint8_t *p, const uint8_t *end, uint64_t lo, uint64_t hi,
+                               const Out *out, int B)
+{
+    int32_t *d = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
+    int rc = 0;
+    for (uint64_t i = lo; i < hi; i += (uint64_t)B) {
+        uint64_t len = (i + (uint64_t)B > hi) ? hi - i : (uint64_t)B;
+        if (p + 4 > end) { rc = -1; break; }
+        int32_t prior;
+        memcpy(&prior, p, 4);
+        p += 4;
+        put_val(out, i, prior);
+        if (len == 1) continue;
+        uint64_t L = len - 1;
+        if (p >= end || p[0] > 32 || p + szpo_block_bytes(p[0], L) > end) { rc = -1; break; }
+        p += szpo_decode_block(p, L, d);
+        for (uint64_t j = 0; j < L; ++j) {
+            prior = (int32_t)((uint32_t)prior + (uint32_t)d[j]);
+            put_val(out, i + 1 + j, prior);
+        }
+    }
+    free(d);
+    return rc;
+}
+
+static int decompress_body(const uint8_t *stream, uint64_t len, uint64_t es, uint64_t n, int B,
+                           int T, int workers, const Out *out)
 {
     if (B < 1 || T < 1) return -1;
-    uint64_t hdr = 4 + 8 * (uint64_t)T;
+    uint64_t hdr = es + 8 * (uint64_t)T;
     if (len < hdr) return -1;
     if (workers <= 0) workers = T;
-    float eb = get_be_f32(stream);
     const uint8_t *body = stream + hdr;
     const uint8_t *end = stream + len;
     int fail = 0;
@@ -330,11 +478,29 @@ int szpo_decompress(const uint8_t *stream, uint64_t len, uint64_t n, int B, int 
     for (int t = 0; t < T; ++t) {
         uint64_t lo, hi, off;
         szpo_chunk(n, T, t, &lo, &hi);
-        memcpy(&off, stream + 4 + 8 * (uint64_t)t, 8);
+        memcpy(&off, stream + es + 8 * (uint64_t)t, 8);
         if (off > len - hdr) { fail = 1; continue; }
-        if (decompress_chunk(body + off, end, lo, hi, eb, B, out)) fail = 1;
+        int rc = out->fast == 2 ? decompress_chunk_ra(body + off, end, lo, hi, out, B)
+                                : decompress_chunk(body + off, end, lo, hi, out, B);
+        if (rc) fail = 1;
     }
     return fail ? -1 : 0;
+}
+
+int szpo_decompress(const uint8_t *stream, uint64_t len, uint64_t n, int B, int T, int workers,
+                    float *out)
+{
+    if (len < 4) return -1;
+    Out o = {out, NULL, get_be_f32(stream), 0.0, 1};
+    return decompress_body(stream, len, 4, n, B, T, workers, &o);
+}
+
+int szpo_decompress_f64(const uint8_t *stream, uint64_t len, uint64_t n, int B, int T,
+                        int workers, double *out)
+{
+    if (len < 8) return -1;
+    Out o = {NULL, out, 0.0f, get_be_f64(stream), 1};
+    return decompress_body(stream, len, 8, n, B, T, workers, &o);
 }
 
 /* SZp/src/szp.cc:29-58 (float, fastMode 1). */
@@ -347,4 +513,87 @@ int szpo_compress_api(const float *x, uint64_t n, int err_mode, double abs_eb, d
         eb = szpo_rel_bound(x, n, rel);
     }
     return szpo_compress(x, n, eb, B, T, workers, out, cap, out_size);
+}
+
+/* SZp/src/szp.cc:60-84: REL range in double, eb = range * rel. */
+double szpo_rel_bound_f64(const double *x, uint64_t n, double rel)
+{
+    double mn = x[0], mx = x[0];
+    for (uint64_t i = 0; i < n; ++i) {
+        double v = x[i];
+        if (mn > v) mn = v;
+        else if (mx < v) mx = v;
+    }
+    double range = mx - mn;
+    return range * rel;
+}
+
+/* szp_compress(fastMode=1, SZ_DOUBLE, ...) semantics (SZp/src/szp.cc:60-98). */
+int szpo_compress_api_f64(const double *x, uint64_t n, int err_mode, double abs_eb, double rel,
+                          int B, int T, int workers, uint8_t *out, uint64_t cap,
+                          uint64_t *out_size)
+{
+    double eb = abs_eb;
+    if (err_mode == 1) {
+        if (n == 0) return -1;
+        eb = szpo_rel_bound_f64(x, n, rel);
+    }
+    return szpo_compress_f64(x, n, eb, B, T, workers, out, cap, out_size);
+}
+
+/* szp_compress(fast, dtype, ...) for fastMode 1 or 2 and SZ_FLOAT (0) or
+ * SZ_DOUBLE (1) (SZp/src/szp.cc:23-115). */
+int szpo_compress_mode(int fast, int dtype, const void *x, uint64_t n, int err_mode,
+                       double abs_eb, double rel, int B, int T, int workers, uint8_t *out,
+                       uint64_t cap, uint64_t *out_size)
+{
+    if ((fast != 1 && fast != 2) || (dtype != 0 && dtype != 1)) return -1;
+    if (B < 1 || T < 1 || !out || !out_size) return -1;
+    if (err_mode == 1 && n == 0) return -1;
+    uint64_t es = dtype ? 8 : 4;
+    if (cap < es + 8 * (uint64_t)T) return -1;
+    Field f = {NULL, NULL, 0.0f, 0.0, fast};
+    if (dtype == 0) {
+        float eb = err_mode == 1 ? szpo_rel_bound((const float *)x, n, rel) : (float)abs_eb;
+        f.f = (const float *)x;
+        f.inv_f = szpo_inverse_bound(eb);
+        put_be_f32(out, eb);
+    } else {
+        double eb = err_mode == 1 ? szpo_rel_bound_f64((const double *)x, n, rel) : abs_eb;
+        volatile double one = 1.0;
+        f.d = (const double *)x;
+        f.inv_d = one / eb;
+        put_be_f64(out, eb);
+    }
+    return compress_body(&f, n, es, B, T, workers, out, cap, out_size);
+}
+
+/* szp_decompress(fast, dtype, ...) (SZp/src/szp.cc:118-179): fastMode 1
+ * decodes SZP_NONRANDOMACCESS, any other value random access. */
+int szpo_decompress_mode(int fast, int dtype, const uint8_t *stream, uint64_t len, uint64_t n,
+                         int B, int T, int workers, void *out)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    uint64_t es = dtype ? 8 : 4;
+    if (len < es) return -1;
+    Out o = {NULL, NULL, 0.0f, 0.0, fast == 1 ? 1 : 2};
+    if (dtype == 0) {
+        o.f = (float *)out;
+        o.eb_f = get_be_f32(stream);
+    } else {
+        o.d = (double *)out;
+        o.eb_d = get_be_f64(stream);
+    }
+    return decompress_body(stream, len, es, n, B, T, workers, &o);
+}
+
+uint64_t szpo_compress_bound_mode(int fast, int dtype, uint64_t n, int B, int T)
+{
+    uint64_t total = (dtype ? 8 : 4) + 8 * (uint64_t)T;
+    for (int t = 0; t < T; ++t) {
+        uint64_t lo, hi;
+        szpo_chunk(n, T, t, &lo, &hi);
+        total += chunk_bound(fast, hi - lo, B);
+    }
+    return total;
 }

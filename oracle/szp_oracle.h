@@ -1,5 +1,6 @@
 /*
- * szp_oracle.h -- CPU restatement of SZp fastmode1 float32 compress/decompress.
+ * szp_oracle.h -- CPU restatement of SZp fastmode1 compress/decompress
+ * (float32 and float64).
  *
  * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load this library, and only as the
@@ -71,6 +72,31 @@ int szpo_decompress(const uint8_t *stream, uint64_t len, uint64_t n, int B, int 
 /* szp_compress(fastMode=1, SZ_FLOAT, ...) semantics (SZp/src/szp.cc:23-58). */
 int szpo_compress_api(const float *x, uint64_t n, int err_mode, double abs_eb, double rel,
                       int B, int T, int workers, uint8_t *out, uint64_t cap, uint64_t *out_size);
+
+/* ---- float64 (SZp/src/szp_double.cc:148-318, SZp/src/szpd_double.cc:23-106):
+ * the same chunks and blocks, an 8-byte big-endian eb (header es + 8T with
+ * es = 8) and the quantizer (int)(x * (1 / eb)) evaluated in double. */
+int32_t szpo_quantize_f64(double x, double inv);
+double szpo_rel_bound_f64(const double *x, uint64_t n, double rel);
+int szpo_compress_f64(const double *x, uint64_t n, double eb, int B, int T, int workers,
+                      uint8_t *out, uint64_t cap, uint64_t *out_size);
+int szpo_decompress_f64(const uint8_t *stream, uint64_t len, uint64_t n, int B, int T,
+                        int workers, double *out);
+int szpo_compress_api_f64(const double *x, uint64_t n, int err_mode, double abs_eb, double rel,
+                          int B, int T, int workers, uint8_t *out, uint64_t cap,
+                          uint64_t *out_size);
+
+/* ---- any fastMode (1, or 2 = random access: SZp/src/szp_float.cc:752-870,
+ * SZp/src/szpd_float.cc:425-530; each chunk cut into units of B values, a
+ * unit = [q(first) i32][block of the other len - 1 deltas]) and dataType
+ * (0 float, 1 double).  Decompress treats any fast != 1 as random access,
+ * as szp_decompress does (SZp/src/szp.cc:135-151). */
+int szpo_compress_mode(int fast, int dtype, const void *x, uint64_t n, int err_mode,
+                       double abs_eb, double rel, int B, int T, int workers, uint8_t *out,
+                       uint64_t cap, uint64_t *out_size);
+int szpo_decompress_mode(int fast, int dtype, const uint8_t *stream, uint64_t len, uint64_t n,
+                         int B, int T, int workers, void *out);
+uint64_t szpo_compress_bound_mode(int fast, int dtype, uint64_t n, int B, int T);
 
 #ifdef __cplusplus
 }

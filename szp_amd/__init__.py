@@ -1,4 +1,4 @@
-"""szp_amd -- SZp fastmode1 float32 compress/decompress on MI355X.
+"""szp_amd -- SZp compress/decompress (fastMode 1 and 2, float32 and float64) on MI355X.
 
 Python mirror of the reference's C entry points (SZp/include/szp.h:33-36),
 driven through the C ABI of the in-tree library ``libszp_amd.so`` (HIP kernels
@@ -79,6 +79,31 @@ def _load():
         "szp_amd_compress_dev_indexed": (i32, [vp, u64, i32, dbl, dbl, i32, i32, vp, u64, vp, vp,
                                                u64, vp, u64, vp]),
         "szp_amd_decompress_dev_indexed": (i32, [vp, u64, vp, u64, u64, i32, i32, vp, vp, u64, vp]),
+        "szp_double_openmp_threadblock": (vp, [vp, ctypes.POINTER(szt), dbl, szt, i32]),
+        "szp_double_openmp_threadblock_arg": (None, [vp, vp, ctypes.POINTER(szt), dbl, szt, i32]),
+        "szp_double_decompress_openmp_threadblock": (vp, [szt, dbl, i32, vp]),
+        "szp_double_decompress_openmp_threadblock_arg": (None, [vp, szt, dbl, i32, vp]),
+        "szp_float_openmp_threadblock_randomaccess": (vp, [vp, ctypes.POINTER(szt), f32, szt, i32]),
+        "szp_float_openmp_threadblock_randomaccess_arg": (None, [vp, vp, ctypes.POINTER(szt), f32,
+                                                                 szt, i32]),
+        "szp_float_decompress_openmp_threadblock_randomaccess": (vp, [szt, f32, i32, vp]),
+        "szp_float_decompress_openmp_threadblock_randomaccess_arg": (None, [vp, szt, f32, i32, vp]),
+        "szp_double_openmp_threadblock_randomaccess": (vp, [vp, ctypes.POINTER(szt), dbl, szt,
+                                                            i32]),
+        "szp_double_openmp_threadblock_randomaccess_arg": (None, [vp, vp, ctypes.POINTER(szt),
+                                                                  dbl, szt, i32]),
+        "szp_double_decompress_openmp_threadblock_randomaccess": (vp, [szt, dbl, i32, vp]),
+        "szp_double_decompress_openmp_threadblock_randomaccess_arg": (None, [vp, szt, dbl, i32,
+                                                                             vp]),
+        "szp_amd_compress_bound_typed": (u64, [i32, i32, u64, i32, i32]),
+        "szp_amd_compress_workspace_size_typed": (u64, [i32, u64, i32, i32]),
+        "szp_amd_decompress_workspace_size_typed": (u64, [i32, u64, u64, i32, i32]),
+        "szp_amd_index_size_typed": (u64, [i32, u64, i32, i32]),
+        "szp_amd_compress_dev_typed": (i32, [i32, i32, vp, u64, i32, dbl, dbl, i32, i32, vp, u64,
+                                             vp, vp, u64, vp, u64, vp]),
+        "szp_amd_decompress_dev_typed": (i32, [i32, i32, vp, u64, vp, u64, u64, i32, i32, vp, vp,
+                                               u64, vp]),
+        "szp_amd_stream_length_typed": (u64, [i32, i32, vp, u64, i32, i32]),
         "szp_amd_profile_enable": (None, [i32]),
         "szp_amd_profile_count": (i32, []),
         "szp_amd_profile_entry": (i32, [i32, ctypes.c_char_p, i32, ctypes.POINTER(u64),
@@ -128,17 +153,28 @@ def set_threads(n):
     lib.szp_amd_set_threads(int(n))
 
 
-def compress_bound(n, block_size, threads):
-    return int(lib.szp_amd_compress_bound(int(n), int(block_size), int(threads)))
+def compress_bound(n, block_size, threads, dataType=SZ_FLOAT, fastMode=SZP_NONRANDOMACCESS):
+    return int(lib.szp_amd_compress_bound_typed(int(fastMode), int(dataType), int(n),
+                                                int(block_size), int(threads)))
 
 
 def bit_width(m):
     return int(lib.szp_amd_bit_width(int(m)))
 
 
-def stream_length(stream, n, block_size, threads):
+def stream_length(stream, n, block_size, threads, dataType=SZ_FLOAT,
+                  fastMode=SZP_NONRANDOMACCESS):
     s = np.ascontiguousarray(stream, dtype=np.uint8)
-    return int(lib.szp_amd_stream_length(s.ctypes.data, int(n), int(block_size), int(threads)))
+    return int(lib.szp_amd_stream_length_typed(int(fastMode), int(dataType), s.ctypes.data,
+                                               int(n), int(block_size), int(threads)))
+
+
+def _np_type(dataType):
+    if dataType == SZ_FLOAT:
+        return np.float32
+    if dataType == SZ_DOUBLE:
+        return np.float64
+    raise SzpError("unknown dataType %r" % (dataType,))
 
 
 def szp_compress(fastMode, dataType, data, errBoundMode, absErrBound, relBoundRatio, nbEle,
@@ -147,7 +183,7 @@ def szp_compress(fastMode, dataType, data, errBoundMode, absErrBound, relBoundRa
 
     The stream is a numpy uint8 array copied out of the library's malloc'd
     buffer (which is then freed, as the reference's callers do)."""
-    x = np.ascontiguousarray(data, dtype=np.float32)
+    x = np.ascontiguousarray(data, dtype=_np_type(dataType))
     if x.size < nbEle:
         raise ValueError("data holds %d elements, nbEle=%d" % (x.size, nbEle))
     size = ctypes.c_size_t(0)
@@ -164,7 +200,8 @@ def szp_compress(fastMode, dataType, data, errBoundMode, absErrBound, relBoundRa
 
 
 def szp_decompress(fastMode, dataType, bytes_, byteLength, nbEle, blockSize):
-    """Mirror of szp_decompress (SZp/include/szp.h:36): returns float32[nbEle]."""
+    """Mirror of szp_decompress (SZp/include/szp.h:36): returns float32[nbEle]
+    (SZ_FLOAT) or float64[nbEle] (SZ_DOUBLE)."""
     s = np.ascontiguousarray(np.frombuffer(bytes_, dtype=np.uint8) if isinstance(bytes_, (bytes, bytearray)) else bytes_,
                              dtype=np.uint8)
     p = lib.szp_decompress(int(fastMode), int(dataType), s.ctypes.data, int(byteLength),
@@ -173,8 +210,10 @@ def szp_decompress(fastMode, dataType, bytes_, byteLength, nbEle, blockSize):
         raise SzpError("szp_decompress failed: " + _last_error())
     try:
         n = int(nbEle)
-        out = (np.ctypeslib.as_array((ctypes.c_float * n).from_address(p)).copy() if n
-               else np.zeros(0, np.float32))
+        vt = _np_type(dataType)
+        ct = ctypes.c_double if vt == np.float64 else ctypes.c_float
+        out = (np.ctypeslib.as_array((ct * n).from_address(p)).copy() if n
+               else np.zeros(0, vt))
     finally:
         _libc.free(p)
     return out
@@ -183,17 +222,22 @@ def szp_decompress(fastMode, dataType, bytes_, byteLength, nbEle, blockSize):
 class DeviceCodec:
     """Device-resident compress/decompress on torch CUDA(HIP) tensors.
 
+    fast_mode is the reference's fastMode: SZP_NONRANDOMACCESS (1, default)
+    or SZP_RANDOMACCESS (2); float32 and float64 tensors are both served.
     Holds the output and workspace tensors of the last shape it served, so
     repeated calls on one field allocate nothing.  Calls are asynchronous on
     the given torch stream (default: the current stream); `check()` waits and
     raises on a device-side error.
     """
 
-    def __init__(self, block_size=128, threads=None, device=None):
+    def __init__(self, block_size=128, threads=None, device=None, fast_mode=SZP_NONRANDOMACCESS):
         import torch
         self.torch = torch
         self.B = int(block_size)
         self.T = int(threads) if threads else default_threads()
+        self.mode = int(fast_mode)
+        if self.mode not in (SZP_NONRANDOMACCESS, SZP_RANDOMACCESS):
+            raise SzpError("unsupported fastMode %r" % (fast_mode,))
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self._bufs = {}
         self.last_work = None
@@ -217,61 +261,85 @@ class DeviceCodec:
         tensor of index_size bytes) it also writes the side index and returns
         (stream_tensor, size_tensor, index_tensor)."""
         torch = self.torch
-        assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous()
+        assert x.is_cuda and x.dtype in (torch.float32, torch.float64) and x.is_contiguous()
+        dt = SZ_DOUBLE if x.dtype == torch.float64 else SZ_FLOAT
         n = x.numel()
-        cap = compress_bound(n, self.B, self.T)
-        wsb = lib.szp_amd_compress_workspace_size(n, self.B, self.T)
+        cap = compress_bound(n, self.B, self.T, dt, self.mode)
+        wsb = lib.szp_amd_compress_workspace_size_typed(self.mode, n, self.B, self.T)
         if out is None:
             out = self._buf("cout", cap)
         size = self._buf("csize", 256).view(torch.int64)
         work = self._buf("cwork", wsb)
         self.last_work = work
-        if index is None or index is False:
-            rc = lib.szp_amd_compress_dev(x.data_ptr(), n, int(err_mode), float(abs_eb), float(rel),
-                                          self.B, self.T, out.data_ptr(), out.numel(),
-                                          size.data_ptr(), work.data_ptr(), work.numel(),
-                                          self._stream(stream))
-            if rc:
-                raise SzpError("szp_amd_compress_dev", rc)
-            return out, size[:1]
-        ib = self.index_size(n)
-        idx = self._buf("cidx", ib) if index is True else index
-        rc = lib.szp_amd_compress_dev_indexed(x.data_ptr(), n, int(err_mode), float(abs_eb),
+        idx = None
+        if index is not None and index is not False:
+            idx = self._buf("cidx", self.index_size(n)) if index is True else index
+        if dt == SZ_FLOAT and self.mode == SZP_NONRANDOMACCESS:
+            # The float fastMode-1 entry points of the device API.
+            if idx is None:
+                rc = lib.szp_amd_compress_dev(x.data_ptr(), n, int(err_mode), float(abs_eb),
                                               float(rel), self.B, self.T, out.data_ptr(),
-                                              out.numel(), size.data_ptr(), idx.data_ptr(),
-                                              idx.numel(), work.data_ptr(), work.numel(),
-                                              self._stream(stream))
+                                              out.numel(), size.data_ptr(), work.data_ptr(),
+                                              work.numel(), self._stream(stream))
+            else:
+                rc = lib.szp_amd_compress_dev_indexed(x.data_ptr(), n, int(err_mode),
+                                                      float(abs_eb), float(rel), self.B, self.T,
+                                                      out.data_ptr(), out.numel(),
+                                                      size.data_ptr(), idx.data_ptr(),
+                                                      idx.numel(), work.data_ptr(), work.numel(),
+                                                      self._stream(stream))
+        else:
+            rc = lib.szp_amd_compress_dev_typed(self.mode, dt, x.data_ptr(), n, int(err_mode),
+                                                float(abs_eb), float(rel), self.B, self.T,
+                                                out.data_ptr(), out.numel(), size.data_ptr(),
+                                                idx.data_ptr() if idx is not None else None,
+                                                idx.numel() if idx is not None else 0,
+                                                work.data_ptr(), work.numel(),
+                                                self._stream(stream))
         if rc:
-            raise SzpError("szp_amd_compress_dev_indexed", rc)
-        return out, size[:1], idx
+            raise SzpError("compress", rc)
+        return (out, size[:1]) if idx is None else (out, size[:1], idx)
 
     def index_size(self, n):
-        return int(lib.szp_amd_index_size(int(n), self.B, self.T))
+        return int(lib.szp_amd_index_size_typed(self.mode, int(n), self.B, self.T))
 
-    def decompress(self, stream_t, stream_bytes, n, out=None, stream=None, index=None):
+    def decompress(self, stream_t, stream_bytes, n, out=None, stream=None, index=None,
+                   dtype=None):
         """Decodes stream_t[:stream_bytes] (stream_bytes may be any upper bound
         inside the tensor).  With `index` (from compress(index=...)) the block
-        boundaries and carries come from the side index instead of a parse."""
+        boundaries and carries come from the side index instead of a parse.
+        dtype (torch.float32 default, or torch.float64) is the field's type;
+        `out`, when given, fixes it."""
         torch = self.torch
         assert stream_t.is_cuda and stream_t.dtype == torch.uint8
-        wsb = lib.szp_amd_decompress_workspace_size(int(stream_bytes), int(n), self.B, self.T)
+        wsb = lib.szp_amd_decompress_workspace_size_typed(self.mode, int(stream_bytes), int(n),
+                                                          self.B, self.T)
         work = self._buf("dwork", wsb)
         self.last_work = work
         if out is None:
-            out = torch.empty(int(n), dtype=torch.float32, device=self.device)
-        if index is None:
-            rc = lib.szp_amd_decompress_dev(stream_t.data_ptr(), int(stream_bytes), int(n), self.B,
-                                            self.T, out.data_ptr(), work.data_ptr(), work.numel(),
-                                            self._stream(stream))
-            if rc:
-                raise SzpError("szp_amd_decompress_dev", rc)
-            return out
-        rc = lib.szp_amd_decompress_dev_indexed(stream_t.data_ptr(), int(stream_bytes),
-                                                index.data_ptr(), index.numel(), int(n), self.B,
-                                                self.T, out.data_ptr(), work.data_ptr(),
+            out = torch.empty(int(n), dtype=dtype or torch.float32, device=self.device)
+        dt = SZ_DOUBLE if out.dtype == torch.float64 else SZ_FLOAT
+        if dt == SZ_FLOAT and self.mode == SZP_NONRANDOMACCESS:
+            if index is None:
+                rc = lib.szp_amd_decompress_dev(stream_t.data_ptr(), int(stream_bytes), int(n),
+                                                self.B, self.T, out.data_ptr(), work.data_ptr(),
                                                 work.numel(), self._stream(stream))
+            else:
+                rc = lib.szp_amd_decompress_dev_indexed(stream_t.data_ptr(), int(stream_bytes),
+                                                        index.data_ptr(), index.numel(), int(n),
+                                                        self.B, self.T, out.data_ptr(),
+                                                        work.data_ptr(), work.numel(),
+                                                        self._stream(stream))
+        else:
+            rc = lib.szp_amd_decompress_dev_typed(self.mode, dt, stream_t.data_ptr(),
+                                                  int(stream_bytes),
+                                                  index.data_ptr() if index is not None else None,
+                                                  index.numel() if index is not None else 0,
+                                                  int(n), self.B, self.T, out.data_ptr(),
+                                                  work.data_ptr(), work.numel(),
+                                                  self._stream(stream))
         if rc:
-            raise SzpError("szp_amd_decompress_dev_indexed", rc)
+            raise SzpError("decompress", rc)
         return out
 
     def check(self, stream=None):

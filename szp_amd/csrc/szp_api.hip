@@ -92,49 +92,65 @@ uint64_t read_u64(const unsigned char *p)
     return v;
 }
 
-// Length of the part of a stream after the 4-byte eb, from the chunk offsets
-// and a walk over the last chunk's width bytes; 0 when malformed.
-uint64_t body_length(const unsigned char *body, uint64_t n, int B, int T)
+// Length of the part of a stream after the eb (4 bytes for float, 8 for
+// double), from the chunk offsets and a walk over the last chunk's width
+// bytes; 0 when malformed.  fastMode 2 walks the last chunk's units, each a
+// 4-byte head and a block of len - 1 values (SZp/src/szpd_float.cc:470-520).
+uint64_t body_length(const unsigned char *body, uint64_t n, int B, int T, int fast = 1)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g)) return 0;
+    if (!make_geom(n, B, T, &g, 4, (uint32_t)fast)) return 0;
     const uint64_t hdr = 8ull * T;
     if (n == 0) return hdr;
-    uint64_t pos = hdr + read_u64(body + hdr - 8) + 4;
     const uint64_t nl = n - (uint64_t)(T - 1) * g.tbs;
+    if (fast == 2) {
+        uint64_t pos = hdr + read_u64(body + hdr - 8);
+        for (uint64_t k = 0; k < g.nblast; ++k) {
+            const uint64_t rem = nl - k * (uint64_t)B;
+            const uint64_t L = (rem < (uint64_t)B ? rem : (uint64_t)B) - 1;
+            pos += 4;
+            const uint32_t b = body[pos];
+            if (L && b > 32) return 0;
+            pos += L ? block_bytes(b, (uint32_t)L) : 1;
+        }
+        return pos;
+    }
+    uint64_t pos = hdr + read_u64(body + hdr - 8) + 4;
     for (uint64_t k = 0; k < g.nblast; ++k) {
         const uint32_t b = body[pos];
         if (b > 32) return 0;
         const uint64_t L = (k + 1 == g.nblast) ? nl - 1 - k * (uint64_t)B : (uint64_t)B;
-        if (!b) {
-            pos += 1;
-        } else {
-            pos += 1 + (L + 7) / 8 + (uint64_t)(b / 8) * L + ((uint64_t)(b % 8) * L + 7) / 8;
-        }
+        pos += block_bytes(b, (uint32_t)L);
     }
     return pos;
 }
 
-// Compress host data through the device; returns 0 and fills *stream (malloc'd
-// unless `into` is given) or a status code.
-int host_compress(const float *data, uint64_t n, int mode, double abs_eb, double rel, int B, int T,
-                  unsigned char *into, unsigned char **stream, size_t *out_size)
+// Bytes per value of a dataType (SZ_FLOAT 4, SZ_DOUBLE 8).
+uint64_t value_bytes(int dtype) { return dtype == SZ_DOUBLE ? 8 : 4; }
+
+// Compress host data of type dtype through the device with fastMode `fast`;
+// returns 0 and fills *stream (malloc'd unless `into` is given) or a status
+// code.
+int host_compress(int fast, int dtype, const void *data, uint64_t n, int mode, double abs_eb,
+                  double rel, int B, int T, unsigned char *into, unsigned char **stream,
+                  size_t *out_size)
 {
     Session *s = session_for_current_device();
     if (!s) return SZP_AMD_ERR_HIP;
     std::lock_guard<std::mutex> lk(s->mu);
-    const uint64_t cap = compress_bound(n, B, T);
-    const uint64_t wsb = compress_workspace_bytes(n, B, T);
+    const uint64_t vb = value_bytes(dtype);
+    const uint64_t cap = compress_bound(n, B, T, dtype, fast);
+    const uint64_t wsb = compress_workspace_bytes(n, B, T, fast);
     if (!cap || !wsb) return SZP_AMD_ERR_ARG;
-    float *d_in = (float *)s->ensure(0, n * 4);
+    void *d_in = s->ensure(0, n * vb);
     unsigned char *d_out = (unsigned char *)s->ensure(1, cap);
     void *d_ws = s->ensure(2, wsb);
     uint64_t *d_size = (uint64_t *)s->ensure(3, 256);
     if (!d_in || !d_out || !d_ws || !d_size) return SZP_AMD_ERR_NOMEM;
-    if (n && hipMemcpyAsync(d_in, data, n * 4, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    if (n && hipMemcpyAsync(d_in, data, n * vb, hipMemcpyHostToDevice, s->stream) != hipSuccess)
         return SZP_AMD_ERR_HIP;
-    int rc = launch_compress(d_in, n, mode, abs_eb, rel, B, T, d_out, cap, d_size, d_ws, wsb,
-                             s->stream);
+    int rc = launch_compress(dtype, d_in, n, mode, abs_eb, rel, B, T, d_out, cap, d_size, d_ws,
+                             wsb, s->stream, nullptr, 0, fast);
     if (rc) return rc;
     rc = szp_amd_check_status(d_ws, s->stream);
     if (rc) return rc;
@@ -151,33 +167,35 @@ int host_compress(const float *data, uint64_t n, int mode, double abs_eb, double
     return SZP_AMD_OK;
 }
 
-// Decompress a host stream given as (eb bytes, body) into `into` or a new
+// Decompress a host stream of type dtype and fastMode `fast` given as (eb
+// bytes, big-endian, value_bytes(dtype) of them; body) into `into` or a new
 // malloc'd array.
-int host_decompress(const unsigned char eb_be[4], const unsigned char *body, uint64_t n, int B,
-                    int T, float *into, float **result)
+int host_decompress(int fast, int dtype, const unsigned char *eb_be, const unsigned char *body,
+                    uint64_t n, int B, int T, void *into, void **result)
 {
-    const uint64_t blen = body_length(body, n, B, T);
+    const uint64_t vb = value_bytes(dtype);
+    const uint64_t blen = body_length(body, n, B, T, fast);
     if (!blen) return SZP_AMD_ERR_STREAM;
-    const uint64_t len = 4 + blen;
+    const uint64_t len = vb + blen;
     Session *s = session_for_current_device();
     if (!s) return SZP_AMD_ERR_HIP;
     std::lock_guard<std::mutex> lk(s->mu);
-    const uint64_t wsb = decompress_workspace_bytes(len, n, B, T);
+    const uint64_t wsb = decompress_workspace_bytes(len, n, B, T, fast);
     if (!wsb) return SZP_AMD_ERR_ARG;
     unsigned char *d_st = (unsigned char *)s->ensure(1, len);
-    float *d_out = (float *)s->ensure(0, n * 4);
+    void *d_out = s->ensure(0, n * vb);
     void *d_ws = s->ensure(2, wsb);
     if (!d_st || !d_out || !d_ws) return SZP_AMD_ERR_NOMEM;
-    if (hipMemcpyAsync(d_st, eb_be, 4, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-        hipMemcpyAsync(d_st + 4, body, blen, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    if (hipMemcpyAsync(d_st, eb_be, vb, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(d_st + vb, body, blen, hipMemcpyHostToDevice, s->stream) != hipSuccess)
         return SZP_AMD_ERR_HIP;
-    int rc = launch_decompress(d_st, len, n, B, T, d_out, d_ws, wsb, s->stream);
+    int rc = launch_decompress(dtype, d_st, len, n, B, T, d_out, d_ws, wsb, s->stream, fast);
     if (rc) return rc;
     rc = szp_amd_check_status(d_ws, s->stream);
     if (rc) return rc;
-    float *dst = into ? into : (float *)malloc(n ? n * 4 : 4);
+    void *dst = into ? into : malloc(n ? n * vb : vb);
     if (!dst) return SZP_AMD_ERR_NOMEM;
-    if (n && hipMemcpy(dst, d_out, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (n && hipMemcpy(dst, d_out, n * vb, hipMemcpyDeviceToHost) != hipSuccess) {
         if (!into) free(dst);
         return SZP_AMD_ERR_HIP;
     }
@@ -185,14 +203,70 @@ int host_decompress(const unsigned char eb_be[4], const unsigned char *body, uin
     return SZP_AMD_OK;
 }
 
+// floatToBytes / doubleToBytes (SZp/include/szp_CompressionToolkit.h:520-546):
+// the value's bytes, most significant first.
 void float_to_be(float v, unsigned char b[4])
 {
     uint32_t u;
     memcpy(&u, &v, 4);
-    b[0] = (unsigned char)(u >> 24);
-    b[1] = (unsigned char)(u >> 16);
-    b[2] = (unsigned char)(u >> 8);
-    b[3] = (unsigned char)u;
+    for (int i = 0; i < 4; ++i) b[i] = (unsigned char)(u >> (24 - 8 * i));
+}
+
+void double_to_be(double v, unsigned char b[8])
+{
+    uint64_t u;
+    memcpy(&u, &v, 8);
+    for (int i = 0; i < 8; ++i) b[i] = (unsigned char)(u >> (56 - 8 * i));
+}
+
+// Shared body of the lower-level compress entry points (szp_float.h /
+// szp_double.h forms): ABS bound, T from szp_amd_default_threads(); writes
+// into `into` when given (the _arg forms), else returns a malloc'd stream.
+unsigned char *lower_compress(const char *who, int fast, int dtype, const void *data,
+                              size_t *outSize, double eb, size_t nbEle, int blockSize,
+                              unsigned char *into)
+{
+    if (!outSize || (!data && nbEle)) {
+        set_error(who, SZP_AMD_ERR_ARG);
+        return NULL;
+    }
+    unsigned char *stream = NULL;
+    size_t size = 0;
+    const int rc = host_compress(fast, dtype, data, nbEle, ABS, eb, 0.0, blockSize,
+                                 szp_amd_default_threads(), into, &stream, &size);
+    if (rc) {
+        set_error(who, rc);
+        *outSize = 0;
+        return NULL;
+    }
+    *outSize = size;
+    return stream;
+}
+
+// Shared body of the lower-level decompress entry points (szpd_float.h /
+// szpd_double.h forms): cmpBytes points just after the eb, which the caller
+// passes as a value; writes into `into` when given (the _arg forms), else
+// returns a malloc'd array.
+void *lower_decompress(const char *who, int fast, int dtype, double eb, size_t nbEle,
+                       int blockSize, unsigned char *cmpBytes, void *into)
+{
+    if (!cmpBytes) {
+        set_error(who, SZP_AMD_ERR_ARG);
+        return NULL;
+    }
+    unsigned char ebb[8];
+    if (dtype == SZ_DOUBLE)
+        double_to_be(eb, ebb);
+    else
+        float_to_be((float)eb, ebb);
+    void *out = NULL;
+    const int rc = host_decompress(fast, dtype, ebb, cmpBytes, nbEle, blockSize,
+                                   szp_amd_default_threads(), into, &out);
+    if (rc) {
+        set_error(who, rc);
+        return NULL;
+    }
+    return out;
 }
 
 }  // namespace
@@ -208,7 +282,7 @@ const char *szp_amd_status_string(int status)
     case SZP_AMD_ERR_HIP: return "HIP runtime error";
     case SZP_AMD_ERR_STREAM: return "malformed compressed stream";
     case SZP_AMD_ERR_TIMEOUT: return "device wait timed out";
-    case SZP_AMD_ERR_UNSUPPORTED: return "only fastMode 1 (SZP_NONRANDOMACCESS) float32 is supported";
+    case SZP_AMD_ERR_UNSUPPORTED: return "only fastMode 1 or 2 with SZ_FLOAT or SZ_DOUBLE is supported";
     case SZP_AMD_ERR_NOMEM: return "allocation failed";
     default: return "unknown status";
     }
@@ -250,7 +324,40 @@ int szp_amd_profile_entry(int index, char *name, int nameCapacity, uint64_t *lau
 
 uint64_t szp_amd_compress_bound(uint64_t nbEle, int blockSize, int nbThreads)
 {
-    return compress_bound(nbEle, blockSize, nbThreads);
+    return compress_bound(nbEle, blockSize, nbThreads, SZ_FLOAT);
+}
+
+static bool typed_ok(int fastMode, int dataType)
+{
+    return (fastMode == SZP_NONRANDOMACCESS || fastMode == SZP_RANDOMACCESS) &&
+           (dataType == SZ_FLOAT || dataType == SZ_DOUBLE);
+}
+
+uint64_t szp_amd_compress_bound_typed(int fastMode, int dataType, uint64_t nbEle, int blockSize,
+                                      int nbThreads)
+{
+    if (!typed_ok(fastMode, dataType)) return 0;
+    return compress_bound(nbEle, blockSize, nbThreads, dataType, fastMode);
+}
+
+uint64_t szp_amd_compress_workspace_size_typed(int fastMode, uint64_t nbEle, int blockSize,
+                                               int nbThreads)
+{
+    if (!typed_ok(fastMode, SZ_FLOAT)) return 0;
+    return compress_workspace_bytes(nbEle, blockSize, nbThreads, fastMode);
+}
+
+uint64_t szp_amd_decompress_workspace_size_typed(int fastMode, uint64_t streamBytes,
+                                                 uint64_t nbEle, int blockSize, int nbThreads)
+{
+    if (!typed_ok(fastMode, SZ_FLOAT)) return 0;
+    return decompress_workspace_bytes(streamBytes, nbEle, blockSize, nbThreads, fastMode);
+}
+
+uint64_t szp_amd_index_size_typed(int fastMode, uint64_t nbEle, int blockSize, int nbThreads)
+{
+    if (!typed_ok(fastMode, SZ_FLOAT)) return 0;
+    return index_bytes(nbEle, blockSize, nbThreads, fastMode);
 }
 
 uint64_t szp_amd_compress_workspace_size(uint64_t nbEle, int blockSize, int nbThreads)
@@ -269,8 +376,8 @@ int szp_amd_compress_dev(const float *d_data, uint64_t nbEle, int errBoundMode,
                          unsigned char *d_out, uint64_t outCapacity, uint64_t *d_outSize,
                          void *d_work, uint64_t workBytes, void *stream)
 {
-    return launch_compress(d_data, nbEle, errBoundMode, absErrBound, relBoundRatio, blockSize,
-                           nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
+    return launch_compress(SZ_FLOAT, d_data, nbEle, errBoundMode, absErrBound, relBoundRatio,
+                           blockSize, nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
                            (hipStream_t)stream);
 }
 
@@ -278,8 +385,8 @@ int szp_amd_decompress_dev(const unsigned char *d_stream, uint64_t streamBytes, 
                            int blockSize, int nbThreads, float *d_out, void *d_work,
                            uint64_t workBytes, void *stream)
 {
-    return launch_decompress(d_stream, streamBytes, nbEle, blockSize, nbThreads, d_out, d_work,
-                             workBytes, (hipStream_t)stream);
+    return launch_decompress(SZ_FLOAT, d_stream, streamBytes, nbEle, blockSize, nbThreads, d_out,
+                             d_work, workBytes, (hipStream_t)stream);
 }
 
 uint64_t szp_amd_index_size(uint64_t nbEle, int blockSize, int nbThreads)
@@ -294,8 +401,8 @@ int szp_amd_compress_dev_indexed(const float *d_data, uint64_t nbEle, int errBou
                                  void *d_work, uint64_t workBytes, void *stream)
 {
     if (!d_index) return SZP_AMD_ERR_ARG;
-    return launch_compress(d_data, nbEle, errBoundMode, absErrBound, relBoundRatio, blockSize,
-                           nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
+    return launch_compress(SZ_FLOAT, d_data, nbEle, errBoundMode, absErrBound, relBoundRatio,
+                           blockSize, nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
                            (hipStream_t)stream, d_index, indexBytes);
 }
 
@@ -304,8 +411,35 @@ int szp_amd_decompress_dev_indexed(const unsigned char *d_stream, uint64_t strea
                                    int blockSize, int nbThreads, float *d_out, void *d_work,
                                    uint64_t workBytes, void *stream)
 {
-    return launch_decompress_indexed(d_stream, streamBytes, d_index, indexBytes, nbEle, blockSize,
-                                     nbThreads, d_out, d_work, workBytes, (hipStream_t)stream);
+    return launch_decompress_indexed(SZ_FLOAT, d_stream, streamBytes, d_index, indexBytes, nbEle,
+                                     blockSize, nbThreads, d_out, d_work, workBytes,
+                                     (hipStream_t)stream);
+}
+
+int szp_amd_compress_dev_typed(int fastMode, int dataType, const void *d_data, uint64_t nbEle,
+                               int errBoundMode, double absErrBound, double relBoundRatio,
+                               int blockSize, int nbThreads, unsigned char *d_out,
+                               uint64_t outCapacity, uint64_t *d_outSize, void *d_index,
+                               uint64_t indexBytes, void *d_work, uint64_t workBytes, void *stream)
+{
+    if (!typed_ok(fastMode, dataType)) return SZP_AMD_ERR_UNSUPPORTED;
+    return launch_compress(dataType, d_data, nbEle, errBoundMode, absErrBound, relBoundRatio,
+                           blockSize, nbThreads, d_out, outCapacity, d_outSize, d_work, workBytes,
+                           (hipStream_t)stream, d_index, indexBytes, fastMode);
+}
+
+int szp_amd_decompress_dev_typed(int fastMode, int dataType, const unsigned char *d_stream,
+                                 uint64_t streamBytes, const void *d_index, uint64_t indexBytes,
+                                 uint64_t nbEle, int blockSize, int nbThreads, void *d_out,
+                                 void *d_work, uint64_t workBytes, void *stream)
+{
+    if (!typed_ok(fastMode, dataType)) return SZP_AMD_ERR_UNSUPPORTED;
+    if (d_index)
+        return launch_decompress_indexed(dataType, d_stream, streamBytes, d_index, indexBytes,
+                                         nbEle, blockSize, nbThreads, d_out, d_work, workBytes,
+                                         (hipStream_t)stream, fastMode);
+    return launch_decompress(dataType, d_stream, streamBytes, nbEle, blockSize, nbThreads, d_out,
+                             d_work, workBytes, (hipStream_t)stream, fastMode);
 }
 
 static_assert(sizeof(szp_amd::BatchPlan) <= sizeof(szp_amd_batch), "batch plan size");
@@ -359,11 +493,23 @@ uint64_t szp_amd_stream_length(const unsigned char *bytes, uint64_t nbEle, int b
     return b ? 4 + b : 0;
 }
 
+uint64_t szp_amd_stream_length_typed(int fastMode, int dataType, const unsigned char *bytes,
+                                     uint64_t nbEle, int blockSize, int nbThreads)
+{
+    if (!bytes || !typed_ok(fastMode, dataType)) return 0;
+    const uint64_t vb = value_bytes(dataType);
+    const uint64_t b = body_length(bytes + vb, nbEle, blockSize, nbThreads, fastMode);
+    return b ? vb + b : 0;
+}
+
 unsigned char *szp_compress(int fastMode, int dataType, void *data, size_t *outSize,
                             int errBoundMode, double absErrBound, double relBoundRatio,
                             size_t nbEle, int blockSize)
 {
-    if (fastMode != SZP_NONRANDOMACCESS || dataType != SZ_FLOAT) {
+    // SZp/src/szp.cc:23-115: fastMode 1 or 2, SZ_FLOAT or SZ_DOUBLE; any
+    // other fastMode returns NULL.
+    if ((fastMode != SZP_NONRANDOMACCESS && fastMode != SZP_RANDOMACCESS) ||
+        (dataType != SZ_FLOAT && dataType != SZ_DOUBLE)) {
         set_error("szp_compress", SZP_AMD_ERR_UNSUPPORTED);
         return NULL;
     }
@@ -373,7 +519,7 @@ unsigned char *szp_compress(int fastMode, int dataType, void *data, size_t *outS
     }
     unsigned char *stream = NULL;
     size_t size = 0;
-    const int rc = host_compress((const float *)data, nbEle, errBoundMode, absErrBound,
+    const int rc = host_compress(fastMode, dataType, data, nbEle, errBoundMode, absErrBound,
                                  relBoundRatio, blockSize, szp_amd_default_threads(), NULL,
                                  &stream, &size);
     if (rc) {
@@ -388,7 +534,7 @@ void *szp_decompress(int fastMode, int dataType, unsigned char *bytes, size_t by
                      size_t nbEle, int blockSize)
 {
     (void)byteLength;  // unused by the reference too (SZp/src/szp.cc:118-153)
-    if (fastMode != SZP_NONRANDOMACCESS || dataType != SZ_FLOAT) {
+    if (dataType != SZ_FLOAT && dataType != SZ_DOUBLE) {
         set_error("szp_decompress", SZP_AMD_ERR_UNSUPPORTED);
         return NULL;
     }
@@ -397,9 +543,12 @@ void *szp_decompress(int fastMode, int dataType, unsigned char *bytes, size_t by
         return NULL;
     }
     szp_sysEndianType = LITTLE_ENDIAN_SYSTEM;  // SZp/src/szp.cc:120-125 on x86-64
-    float *out = NULL;
-    const int rc = host_decompress(bytes, bytes + 4, nbEle, blockSize, szp_amd_default_threads(),
-                                   NULL, &out);
+    // Every fastMode other than 1 decodes as random access (SZp/src/szp.cc:135-151).
+    const int fast = fastMode == SZP_NONRANDOMACCESS ? 1 : 2;
+    void *out = NULL;
+    const uint64_t vb = value_bytes(dataType);
+    const int rc = host_decompress(fast, dataType, bytes, bytes + vb, nbEle, blockSize,
+                                   szp_amd_default_threads(), NULL, &out);
     if (rc) {
         set_error("szp_decompress", rc);
         return NULL;
@@ -407,76 +556,160 @@ void *szp_decompress(int fastMode, int dataType, unsigned char *bytes, size_t by
     return out;
 }
 
+// ---- lower-level entry points (szp_float.h, szpd_float.h, szp_double.h,
+// szpd_double.h): ABS bound given as the value type, T from
+// szp_amd_default_threads().
+
 unsigned char *szp_float_openmp_threadblock(float *oriData, size_t *outSize, float absErrBound,
                                             size_t nbEle, int blockSize)
 {
-    if (!outSize || (!oriData && nbEle)) {
-        set_error("szp_float_openmp_threadblock", SZP_AMD_ERR_ARG);
-        return NULL;
-    }
-    unsigned char *stream = NULL;
-    size_t size = 0;
-    const int rc = host_compress(oriData, nbEle, ABS, (double)absErrBound, 0.0, blockSize,
-                                 szp_amd_default_threads(), NULL, &stream, &size);
-    if (rc) {
-        set_error("szp_float_openmp_threadblock", rc);
-        return NULL;
-    }
-    *outSize = size;
-    return stream;
+    return lower_compress("szp_float_openmp_threadblock", 1, SZ_FLOAT, oriData, outSize,
+                          (double)absErrBound, nbEle, blockSize, NULL);
 }
 
 void szp_float_openmp_threadblock_arg(unsigned char *output, float *oriData, size_t *outSize,
                                       float absErrBound, size_t nbEle, int blockSize)
 {
-    if (!output || !outSize || (!oriData && nbEle)) {
+    if (!output) {
         set_error("szp_float_openmp_threadblock_arg", SZP_AMD_ERR_ARG);
         return;
     }
-    unsigned char *stream = NULL;
-    size_t size = 0;
-    const int rc = host_compress(oriData, nbEle, ABS, (double)absErrBound, 0.0, blockSize,
-                                 szp_amd_default_threads(), output, &stream, &size);
-    if (rc) {
-        set_error("szp_float_openmp_threadblock_arg", rc);
-        *outSize = 0;
-        return;
-    }
-    *outSize = size;
+    lower_compress("szp_float_openmp_threadblock_arg", 1, SZ_FLOAT, oriData, outSize,
+                   (double)absErrBound, nbEle, blockSize, output);
 }
 
 float *szp_float_decompress_openmp_threadblock(size_t nbEle, float absErrBound, int blockSize,
                                                unsigned char *cmpBytes)
 {
-    if (!cmpBytes) {
-        set_error("szp_float_decompress_openmp_threadblock", SZP_AMD_ERR_ARG);
-        return NULL;
-    }
-    unsigned char eb[4];
-    float_to_be(absErrBound, eb);
-    float *out = NULL;
-    const int rc = host_decompress(eb, cmpBytes, nbEle, blockSize, szp_amd_default_threads(), NULL,
-                                   &out);
-    if (rc) {
-        set_error("szp_float_decompress_openmp_threadblock", rc);
-        return NULL;
-    }
-    return out;
+    return (float *)lower_decompress("szp_float_decompress_openmp_threadblock", 1, SZ_FLOAT,
+                                     absErrBound, nbEle, blockSize, cmpBytes, NULL);
 }
 
 void szp_float_decompress_openmp_threadblock_arg(float *newData, size_t nbEle, float absErrBound,
                                                  int blockSize, unsigned char *cmpBytes)
 {
-    if (!cmpBytes || (!newData && nbEle)) {
+    if (!newData && nbEle) {
         set_error("szp_float_decompress_openmp_threadblock_arg", SZP_AMD_ERR_ARG);
         return;
     }
-    unsigned char eb[4];
-    float_to_be(absErrBound, eb);
-    float *out = NULL;
-    const int rc = host_decompress(eb, cmpBytes, nbEle, blockSize, szp_amd_default_threads(),
-                                   newData, &out);
-    if (rc) set_error("szp_float_decompress_openmp_threadblock_arg", rc);
+    lower_decompress("szp_float_decompress_openmp_threadblock_arg", 1, SZ_FLOAT, absErrBound,
+                     nbEle, blockSize, cmpBytes, newData);
+}
+
+unsigned char *szp_float_openmp_threadblock_randomaccess(float *oriData, size_t *outSize,
+                                                         float absErrBound, size_t nbEle,
+                                                         int blockSize)
+{
+    return lower_compress("szp_float_openmp_threadblock_randomaccess", 2, SZ_FLOAT, oriData,
+                          outSize, (double)absErrBound, nbEle, blockSize, NULL);
+}
+
+void szp_float_openmp_threadblock_randomaccess_arg(unsigned char *output, float *oriData,
+                                                   size_t *outSize, float absErrBound,
+                                                   size_t nbEle, int blockSize)
+{
+    if (!output) {
+        set_error("szp_float_openmp_threadblock_randomaccess_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_compress("szp_float_openmp_threadblock_randomaccess_arg", 2, SZ_FLOAT, oriData, outSize,
+                   (double)absErrBound, nbEle, blockSize, output);
+}
+
+float *szp_float_decompress_openmp_threadblock_randomaccess(size_t nbEle, float absErrBound,
+                                                            int blockSize, unsigned char *cmpBytes)
+{
+    return (float *)lower_decompress("szp_float_decompress_openmp_threadblock_randomaccess", 2,
+                                     SZ_FLOAT, absErrBound, nbEle, blockSize, cmpBytes, NULL);
+}
+
+void szp_float_decompress_openmp_threadblock_randomaccess_arg(float *newData, size_t nbEle,
+                                                              float absErrBound, int blockSize,
+                                                              unsigned char *cmpBytes)
+{
+    if (!newData && nbEle) {
+        set_error("szp_float_decompress_openmp_threadblock_randomaccess_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_decompress("szp_float_decompress_openmp_threadblock_randomaccess_arg", 2, SZ_FLOAT,
+                     absErrBound, nbEle, blockSize, cmpBytes, newData);
+}
+
+unsigned char *szp_double_openmp_threadblock(double *oriData, size_t *outSize, double absErrBound,
+                                             size_t nbEle, int blockSize)
+{
+    return lower_compress("szp_double_openmp_threadblock", 1, SZ_DOUBLE, oriData, outSize,
+                          absErrBound, nbEle, blockSize, NULL);
+}
+
+void szp_double_openmp_threadblock_arg(unsigned char *output, double *oriData, size_t *outSize,
+                                       double absErrBound, size_t nbEle, int blockSize)
+{
+    if (!output) {
+        set_error("szp_double_openmp_threadblock_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_compress("szp_double_openmp_threadblock_arg", 1, SZ_DOUBLE, oriData, outSize,
+                   absErrBound, nbEle, blockSize, output);
+}
+
+double *szp_double_decompress_openmp_threadblock(size_t nbEle, double absErrBound, int blockSize,
+                                                 unsigned char *cmpBytes)
+{
+    return (double *)lower_decompress("szp_double_decompress_openmp_threadblock", 1, SZ_DOUBLE,
+                                      absErrBound, nbEle, blockSize, cmpBytes, NULL);
+}
+
+void szp_double_decompress_openmp_threadblock_arg(double *newData, size_t nbEle,
+                                                  double absErrBound, int blockSize,
+                                                  unsigned char *cmpBytes)
+{
+    if (!newData && nbEle) {
+        set_error("szp_double_decompress_openmp_threadblock_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_decompress("szp_double_decompress_openmp_threadblock_arg", 1, SZ_DOUBLE, absErrBound,
+                     nbEle, blockSize, cmpBytes, newData);
+}
+
+unsigned char *szp_double_openmp_threadblock_randomaccess(double *oriData, size_t *outSize,
+                                                          double absErrBound, size_t nbEle,
+                                                          int blockSize)
+{
+    return lower_compress("szp_double_openmp_threadblock_randomaccess", 2, SZ_DOUBLE, oriData,
+                          outSize, absErrBound, nbEle, blockSize, NULL);
+}
+
+void szp_double_openmp_threadblock_randomaccess_arg(unsigned char *output, double *oriData,
+                                                    size_t *outSize, double absErrBound,
+                                                    size_t nbEle, int blockSize)
+{
+    if (!output) {
+        set_error("szp_double_openmp_threadblock_randomaccess_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_compress("szp_double_openmp_threadblock_randomaccess_arg", 2, SZ_DOUBLE, oriData,
+                   outSize, absErrBound, nbEle, blockSize, output);
+}
+
+double *szp_double_decompress_openmp_threadblock_randomaccess(size_t nbEle, double absErrBound,
+                                                              int blockSize,
+                                                              unsigned char *cmpBytes)
+{
+    return (double *)lower_decompress("szp_double_decompress_openmp_threadblock_randomaccess", 2,
+                                      SZ_DOUBLE, absErrBound, nbEle, blockSize, cmpBytes, NULL);
+}
+
+void szp_double_decompress_openmp_threadblock_randomaccess_arg(double *newData, size_t nbEle,
+                                                               double absErrBound, int blockSize,
+                                                               unsigned char *cmpBytes)
+{
+    if (!newData && nbEle) {
+        set_error("szp_double_decompress_openmp_threadblock_randomaccess_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_decompress("szp_double_decompress_openmp_threadblock_randomaccess_arg", 2, SZ_DOUBLE,
+                     absErrBound, nbEle, blockSize, cmpBytes, newData);
 }
 
 }  // extern "C"

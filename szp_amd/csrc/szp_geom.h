@@ -22,20 +22,29 @@ constexpr int kOutCap = 28672;           // LDS bytes buffering a compress tile 
 constexpr int kDecTileBytes = 20480;     // LDS bytes holding a decode tile's stream bytes
 
 // Geometry of the item sequence of one field.  The stream body is the
-// concatenation, in element order, of "items": the raw head value of each
-// non-empty chunk (4 bytes) followed by that chunk's SZp blocks.  A tile is R
-// rounds of K consecutive items; every item owns spb = ceil(B/8) thread slots
-// ("octets").
+// concatenation, in element order, of "items", each either a head (one raw
+// quantized value, 4 bytes) or a block (a width byte and the packed deltas of
+// its values against their predecessors).
+//   fastMode 1 (SZp/src/szp_float.cc:201-275): each non-empty chunk is its
+//     head followed by blocks of B values (the last one shorter).
+//   fastMode 2, "random access" (SZp/src/szp_float.cc:752-870): each chunk is
+//     cut into units of B values from its start; a unit is the head of its
+//     first value followed by one block of the other L = len - 1 values
+//     (L = 0 still writes its zero width byte).
+// A tile is R rounds of K consecutive items; every item owns spb = ceil(B/8)
+// thread slots ("octets").
 struct Geom {
     uint64_t n;        // elements
     uint64_t tbs;      // floor(n / T), the size of chunks 0..T-2
     uint64_t c0;       // items in chunks 0..T-2 (0 when tbs == 0)
     uint64_t clast;    // items in chunk T-1
     uint64_t nitems;   // (T-1)*c0 + clast
-    uint64_t nb0;      // SZp blocks in chunks 0..T-2
-    uint64_t nblast;   // SZp blocks in chunk T-1
+    uint64_t nb0;      // SZp blocks (mode 2: units) in chunks 0..T-2
+    uint64_t nblast;   // SZp blocks (mode 2: units) in chunk T-1
     uint64_t ntiles;   // ceil(nitems / (K * R))
-    uint64_t body;     // 4 + 8T: offset of chunk 0 in the stream
+    uint64_t body;     // es + 8T: offset of chunk 0 in the stream
+    uint32_t es;       // bytes of the error bound at the stream's start (4 float, 8 double)
+    uint32_t mode;     // 1: fastMode 1, 2: fastMode 2 (random access)
     uint32_t T;        // chunks (the reference's OpenMP team size)
     uint32_t B;        // SZp block size
     uint32_t spb;      // slots per item
@@ -44,13 +53,20 @@ struct Geom {
     uint32_t R;          // rounds per tile (1 when chunks are smaller than a tile)
 };
 
-// Returns false when (n, B, T) is outside what the kernels handle.
-inline bool make_geom(uint64_t n, int B, int T, Geom *g)
+// Returns false when (n, B, T, mode) is outside what the kernels handle.  es
+// is the size of the value type: the stream opens with the error bound in
+// that type (SZp/src/szp_float.cc:165, SZp/src/szp_double.cc:165).
+inline bool make_geom(uint64_t n, int B, int T, Geom *g, uint32_t es = 4, uint32_t mode = 1)
 {
-    if (B < 1 || B > kMaxBlock || T < 1) return false;
+    if (B < 1 || B > kMaxBlock || T < 1 || (mode != 1 && mode != 2)) return false;
     const uint64_t b = (uint64_t)B;
-    auto items_of = [&](uint64_t m) -> uint64_t { return m ? 1 + (m - 1 + b - 1) / b : 0; };
-    auto blocks_of = [&](uint64_t m) -> uint64_t { return m ? (m - 1 + b - 1) / b : 0; };
+    auto blocks_of = [&](uint64_t m) -> uint64_t {
+        return mode == 2 ? (m + b - 1) / b : (m ? (m - 1 + b - 1) / b : 0);
+    };
+    auto items_of = [&](uint64_t m) -> uint64_t {
+        return mode == 2 ? 2 * blocks_of(m) : (m ? 1 + blocks_of(m) : 0);
+    };
+    g->mode = mode;
     g->n = n;
     g->T = (uint32_t)T;
     g->B = (uint32_t)B;
@@ -69,7 +85,8 @@ inline bool make_geom(uint64_t n, int B, int T, Geom *g)
     g->R = (g->c0 == 0 || g->c0 * (kWG - 1) >= (uint64_t)g->K * kMaxRounds) ? kMaxRounds : 1;
     const uint64_t per_tile = (uint64_t)g->K * g->R;
     g->ntiles = (g->nitems + per_tile - 1) / per_tile;
-    g->body = 4 + 8ull * (uint64_t)T;
+    g->es = es;
+    g->body = es + 8ull * (uint64_t)T;
     return true;
 }
 
@@ -80,7 +97,7 @@ SZP_HD uint32_t block_bytes(uint32_t b, uint32_t L)
     return 1 + ((L + 7) >> 3) + (b >> 3) * L + (((b & 7) * L + 7) >> 3);
 }
 
-// Item -> (chunk t, index j within the chunk; j == 0 is the chunk's head).
+// Item -> (chunk t, index j within the chunk; j == 0 is the chunk's first head).
 SZP_HD void locate(const Geom &g, uint64_t item, uint32_t &t, uint64_t &j)
 {
     const uint64_t lead = (uint64_t)(g.T - 1) * g.c0;
@@ -110,15 +127,29 @@ SZP_HD uint64_t chunk_len(const Geom &g, uint32_t t)
     return t == g.T - 1 ? g.n - chunk_lo(g, t) : g.tbs;
 }
 
+// Item j of a chunk is a head (a raw quantized value).
+SZP_HD bool item_head(const Geom &g, uint64_t j) { return g.mode == 2 ? !(j & 1) : j == 0; }
+
+// Items whose stream position the block index holds (blocks in mode 1, units
+// -- at their head -- in mode 2), and their number within the chunk.
+SZP_HD bool item_indexed(const Geom &g, uint64_t j) { return g.mode == 2 ? !(j & 1) : j != 0; }
+SZP_HD uint64_t item_block(const Geom &g, uint64_t j) { return g.mode == 2 ? j >> 1 : j - 1; }
+
 // First element of item (t, j).
 SZP_HD uint64_t item_elem(const Geom &g, uint32_t t, uint64_t j)
 {
+    if (g.mode == 2) return chunk_lo(g, t) + (j >> 1) * (uint64_t)g.B + (j & 1);
     return chunk_lo(g, t) + (j ? 1 + (j - 1) * (uint64_t)g.B : 0);
 }
 
 // Elements of item (t, j).
 SZP_HD uint32_t item_len(const Geom &g, uint32_t t, uint64_t j)
 {
+    if (g.mode == 2) {
+        if (!(j & 1)) return 1;
+        const uint64_t rem = chunk_len(g, t) - (j >> 1) * (uint64_t)g.B;
+        return (rem < g.B ? (uint32_t)rem : g.B) - 1;
+    }
     if (!j) return 1;
     const uint64_t rem = chunk_len(g, t) - 1 - (j - 1) * (uint64_t)g.B;
     return rem < g.B ? (uint32_t)rem : g.B;
@@ -138,6 +169,7 @@ SZP_HD uint64_t item_start(const Geom &g, uint64_t item)
 // chunk ct, with B a multiple of 8: every octet then holds 8 values.
 SZP_HD bool simple_tile(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nti)
 {
+    if (g.mode != 1) return false;
     const uint64_t nfull = (chunk_len(g, ct) - 1) / g.B;
     return (g.B & 7) == 0 && cj >= 1 && nti >= 1 && cj + nti - 1 <= nfull;
 }
@@ -166,12 +198,12 @@ struct TileOrder {
 // Tiles holding a chunk head or a chunk's short last block, and with
 // with_last the field's last tile (ascending, each once).  n = 0 when there
 // are more than kMaxGenTiles, or when every tile takes the general path
-// anyway (B % 8 != 0).
+// anyway (B % 8 != 0, or fastMode 2).
 inline TileOrder general_tiles(const Geom &g, bool with_last = false)
 {
     TileOrder o;
     o.n = 0;
-    if (g.B & 7) return o;
+    if ((g.B & 7) || g.mode != 1) return o;
     const uint64_t per_tile = (uint64_t)g.K * g.R;
     auto add = [&](uint64_t tile) -> bool {
         if (o.n && o.t[o.n - 1] == tile) return true;
@@ -215,19 +247,21 @@ struct Oct {
     uint32_t L;   // elements of the item
     uint32_t t;   // chunk of the item
     bool live;    // the slot's item exists in this round
-    bool head;    // the item is a chunk head
+    bool head;    // the item is a head
+    bool first;   // the item is its chunk's first head (its offset goes in the header)
 };
 
 SZP_HD Oct octet_of(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nit, uint32_t d, uint32_t o)
 {
-    Oct s{0, 0, 0, 0, false, false};
+    Oct s{0, 0, 0, 0, false, false, false};
     if (d >= g.K || d >= nit) return s;
     uint32_t t = ct;
     uint64_t j = cj;
     advance(g, t, j, d);
     s.live = true;
     s.t = t;
-    s.head = j == 0;
+    s.head = item_head(g, j);
+    s.first = j == 0;
     s.L = item_len(g, t, j);
     const uint64_t es = item_elem(g, t, j);
     if (s.head) {
@@ -298,6 +332,23 @@ SZP_HD uint32_t zero_run(const uint8_t *buf, uint32_t p, uint32_t v)
 // Length of a block with width byte b and B values.
 SZP_HD uint32_t blen(uint32_t b, uint32_t B) { return block_bytes(b, B); }
 
+// A chain step of the parse.  Mode 1 steps over one block (width byte at p);
+// mode 2 over one unit: 4 head bytes, then a block of B - 1 values whose width
+// byte is at p + 4.  Returns the width byte (> 32: no item can start at p)
+// and sets len to the step.
+SZP_HD uint32_t parse_step(const uint8_t *tb, uint32_t p, uint32_t B, uint32_t mode,
+                           uint32_t &len)
+{
+    if (mode == 2) {
+        const uint32_t b = tb[p + 4];
+        len = 4 + blen(b, B - 1);
+        return b;
+    }
+    const uint32_t b = tb[p];
+    len = blen(b, B);
+    return b;
+}
+
 }  // namespace szp_amd
 
 namespace szp_amd {
@@ -318,13 +369,14 @@ namespace szp_amd {
 // Walks the chain from c; true when it reaches v (exit = its first position
 // >= v), false when it meets a width byte > 32.
 SZP_HD bool parse_chain_leaves(const uint8_t *tb, uint32_t c, uint32_t v, uint32_t B,
-                               uint32_t &exit)
+                               uint32_t &exit, uint32_t mode = 1)
 {
     uint32_t p = c;
     while (p < v) {
-        const uint32_t b = tb[p];
+        uint32_t len;
+        const uint32_t b = parse_step(tb, p, B, mode, len);
         if (b > 32) return false;
-        p += b ? blen(b, B) : zero_run(tb, p, v);
+        p += (b || mode == 2) ? len : zero_run(tb, p, v);
     }
     exit = p;
     return true;
@@ -333,12 +385,17 @@ SZP_HD bool parse_chain_leaves(const uint8_t *tb, uint32_t c, uint32_t v, uint32
 // Marks the block starts of the chain from c (which leaves the sub-tile) in
 // spec; returns their count.
 SZP_HD uint32_t parse_mark_chain(const uint8_t *tb, uint32_t *spec, uint32_t c, uint32_t v,
-                                 uint32_t B)
+                                 uint32_t B, uint32_t mode = 1)
 {
     uint32_t p = c, n = 0;
     while (p < v) {
-        const uint32_t b = tb[p];
-        if (b == 0) {
+        uint32_t len;
+        const uint32_t b = parse_step(tb, p, B, mode, len);
+        if (mode == 2) {
+            bm_set(spec, p);
+            ++n;
+            p += len;
+        } else if (b == 0) {
             const uint32_t k = zero_run(tb, p, v);
             bm_set_range(spec, p, k);
             n += k;
@@ -346,7 +403,7 @@ SZP_HD uint32_t parse_mark_chain(const uint8_t *tb, uint32_t *spec, uint32_t c, 
         } else {
             bm_set(spec, p);
             ++n;
-            p += blen(b, B);
+            p += len;
         }
     }
     return n;
@@ -355,19 +412,20 @@ SZP_HD uint32_t parse_mark_chain(const uint8_t *tb, uint32_t *spec, uint32_t c, 
 // Chain from candidate c: 0 if it joins spec or dies inside [u, v), else the
 // position (>= v) where it leaves.
 SZP_HD uint32_t parse_cand_walk(const uint8_t *tb, const uint32_t *spec, uint32_t c, uint32_t v,
-                                uint32_t B)
+                                uint32_t B, uint32_t mode = 1)
 {
     uint32_t p = c;
     while (true) {
         if (bm_test(spec, p)) return 0;
-        const uint32_t b = tb[p];
+        uint32_t len;
+        const uint32_t b = parse_step(tb, p, B, mode, len);
         if (b > 32) return 0;
-        if (b == 0) {
+        if (b == 0 && mode == 1) {
             const uint32_t k = zero_run(tb, p, v);
             if (bm_first(spec, p, k) < p + k) return 0;
             p += k;
         } else {
-            p += blen(b, B);
+            p += len;
         }
         if (p >= v) return p;
     }
@@ -377,14 +435,15 @@ SZP_HD uint32_t parse_cand_walk(const uint8_t *tb, const uint32_t *spec, uint32_
 // Returns 1 merged (p = the join position), 2 broken (width byte > 32 at p),
 // 0 left the sub-tile (p >= v).  npre counts the marked blocks.
 SZP_HD int parse_true_walk(const uint8_t *tb, const uint32_t *spec, uint32_t *pre, uint32_t &p,
-                           uint32_t v, uint32_t B, uint32_t &npre)
+                           uint32_t v, uint32_t B, uint32_t &npre, uint32_t mode = 1)
 {
     npre = 0;
     while (p < v) {
         if (bm_test(spec, p)) return 1;
-        const uint32_t b = tb[p];
+        uint32_t len;
+        const uint32_t b = parse_step(tb, p, B, mode, len);
         if (b > 32) return 2;
-        if (b == 0) {
+        if (b == 0 && mode == 1) {
             const uint32_t k = zero_run(tb, p, v);
             const uint32_t q = bm_first(spec, p, k);
             bm_set_range(pre, p, q - p);
@@ -397,7 +456,7 @@ SZP_HD int parse_true_walk(const uint8_t *tb, const uint32_t *spec, uint32_t *pr
         } else {
             bm_set(pre, p);
             ++npre;
-            p += blen(b, B);
+            p += len;
         }
     }
     return 0;

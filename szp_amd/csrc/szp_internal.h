@@ -8,9 +8,12 @@
 
 namespace szp_amd {
 
-uint64_t compress_bound(uint64_t n, int B, int T);
-uint64_t compress_workspace_bytes(uint64_t n, int B, int T);
-uint64_t decompress_workspace_bytes(uint64_t stream_bytes, uint64_t n, int B, int T);
+// dtype: 0 float (SZ_FLOAT), 1 double (SZ_DOUBLE); mode: the reference's
+// fastMode, 1 (SZP_NONRANDOMACCESS) or 2 (SZP_RANDOMACCESS).
+uint64_t compress_bound(uint64_t n, int B, int T, int dtype = 0, int mode = 1);
+uint64_t compress_workspace_bytes(uint64_t n, int B, int T, int mode = 1);
+uint64_t decompress_workspace_bytes(uint64_t stream_bytes, uint64_t n, int B, int T,
+                                    int mode = 1);
 
 // Error bits accumulated in the workspace control word.
 enum : uint32_t {
@@ -23,19 +26,21 @@ enum : uint32_t {
 // Return 0 or a positive szp_amd_status code for argument errors.
 // d_index (optional, 256-byte aligned, index_bytes() long) receives the side
 // index: each block's stream position and each tile's carry.
-int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, double rel,
-                    int B, int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size,
+int launch_compress(int dtype, const void *d_x, uint64_t n, int err_mode, double abs_eb,
+                    double rel, int B, int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size,
                     void *d_work, uint64_t work_bytes, hipStream_t stream,
-                    void *d_index = nullptr, uint64_t idx_bytes = 0);
+                    void *d_index = nullptr, uint64_t idx_bytes = 0, int mode = 1);
 
-uint64_t index_bytes(uint64_t n, int B, int T);
+uint64_t index_bytes(uint64_t n, int B, int T, int mode = 1);
 
-int launch_decompress_indexed(const uint8_t *d_stream, uint64_t stream_bytes, const void *d_index,
-                              uint64_t idx_bytes, uint64_t n, int B, int T, float *d_out,
-                              void *d_work, uint64_t work_bytes, hipStream_t stream);
+int launch_decompress_indexed(int dtype, const uint8_t *d_stream, uint64_t stream_bytes,
+                              const void *d_index, uint64_t idx_bytes, uint64_t n, int B, int T,
+                              void *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream,
+                              int mode = 1);
 
-int launch_decompress(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int B, int T,
-                      float *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream);
+int launch_decompress(int dtype, const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int B,
+                      int T, void *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream,
+                      int mode = 1);
 
 // ---- batched forms: many fields in one persistent launch ----
 // One field of a batch.  Compress: in = float[n], out = stream buffer of

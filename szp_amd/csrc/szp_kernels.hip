@@ -87,6 +87,51 @@ __device__ __forceinline__ int32_t quantize(float x, float inv)
     return __builtin_fabs(p) < 2147483648.0 ? (int32_t)p : INT32_MIN;
 }
 
+// (int)(x * inv) for double fields (SZp/src/szp_double.cc:209,228): the
+// product is rounded to double, then truncated with the same x86 rule.
+__device__ __forceinline__ int32_t quantize(double x, double inv)
+{
+    const double p = x * inv;
+    return __builtin_fabs(p) < 2147483648.0 ? (int32_t)p : INT32_MIN;
+}
+
+// min/max that return the non-NaN operand, per value type.
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+
+// The error bound as the stream's first sizeof(V) bytes, big-endian
+// (floatToBytes / doubleToBytes, SZp/include/szp_CompressionToolkit.h:520-546).
+__device__ __forceinline__ void put_eb(uint8_t *out, float eb)
+{
+    const uint32_t u = __float_as_uint(eb);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = (uint8_t)(u >> (24 - 8 * i));
+}
+__device__ __forceinline__ void put_eb(uint8_t *out, double eb)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(eb);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = (uint8_t)(u >> (56 - 8 * i));
+}
+template <typename V>
+__device__ __forceinline__ V get_eb(const uint8_t *st);
+template <>
+__device__ __forceinline__ float get_eb<float>(const uint8_t *st)
+{
+    return __uint_as_float(((uint32_t)st[0] << 24) | ((uint32_t)st[1] << 16) |
+                           ((uint32_t)st[2] << 8) | (uint32_t)st[3]);
+}
+template <>
+__device__ __forceinline__ double get_eb<double>(const uint8_t *st)
+{
+    uint64_t u = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) u = (u << 8) | st[i];
+    return __longlong_as_double((long long)u);
+}
+
 __device__ __forceinline__ uint64_t ld_status(const uint64_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -262,9 +307,10 @@ __global__ void k_clear(uint32_t *ctl, uint64_t *status, uint64_t nstatus)
     for (uint64_t k = i; k < nstatus; k += (uint64_t)gridDim.x * blockDim.x) status[k] = 0;
 }
 
+template <typename V>
 struct RangePart {
-    float mn;
-    float mx;
+    V mn;
+    V mx;
 };
 
 // The serial scan of SZp/src/szp.cc:36-45 keeps the first occurrence of the
@@ -275,14 +321,15 @@ struct RangePart {
 // the other's zero sign cannot change the exact difference, and with max ==
 // min the scan subtracts x[0] from itself, i.e. max - max.  So the GPU keeps
 // no indices: fminf/fmaxf (which return the non-NaN operand) in any order.
-__device__ __forceinline__ void range_reduce_wg(float &mn, float &mx)
+template <typename V>
+__device__ __forceinline__ void range_reduce_wg(V &mn, V &mx)
 {
-    __shared__ float smn[kWG / 64], smx[kWG / 64];
+    __shared__ V smn[kWG / 64], smx[kWG / 64];
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o; o >>= 1) {
-        mn = fminf(mn, __shfl_xor(mn, o));
-        mx = fmaxf(mx, __shfl_xor(mx, o));
+        mn = vmin(mn, __shfl_xor(mn, o));
+        mx = vmax(mx, __shfl_xor(mx, o));
     }
     if (lane == 0) {
         smn[wv] = mn;
@@ -291,10 +338,26 @@ __device__ __forceinline__ void range_reduce_wg(float &mn, float &mx)
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < kWG / 64; ++k) {
-            mn = fminf(mn, smn[k]);
-            mx = fmaxf(mx, smx[k]);
+            mn = vmin(mn, smn[k]);
+            mx = vmax(mx, smx[k]);
         }
     }
+}
+
+// Min and max of one 16-byte vector of values.
+__device__ __forceinline__ void vec_minmax(const uint8_t *p, float &mn, float &mx)
+{
+    float4 q;
+    __builtin_memcpy(&q, p, 16);
+    mn = fminf(mn, fminf(fminf(q.x, q.y), fminf(q.z, q.w)));
+    mx = fmaxf(mx, fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w)));
+}
+__device__ __forceinline__ void vec_minmax(const uint8_t *p, double &mn, double &mx)
+{
+    double2 q;
+    __builtin_memcpy(&q, p, 16);
+    mn = fmin(mn, fmin(q.x, q.y));
+    mx = fmax(mx, fmax(q.x, q.y));
 }
 
 // Min and max of x[s, s + seg) per workgroup.  VEC: x is 16-byte aligned and
@@ -302,61 +365,62 @@ __device__ __forceinline__ void range_reduce_wg(float &mn, float &mx)
 // is ragged).
 constexpr int kRangeUnroll = 4;
 
-template <bool VEC>
-__global__ __launch_bounds__(kWG) void k_range_partial(const float *__restrict__ x, uint64_t n,
-                                                       uint64_t seg, RangePart *part)
+template <typename V, bool VEC>
+__global__ __launch_bounds__(kWG) void k_range_partial(const V *__restrict__ x, uint64_t n,
+                                                       uint64_t seg, RangePart<V> *part)
 {
+    constexpr uint32_t kPer = 16 / sizeof(V);  // values per 16-byte load
     const uint64_t s = (uint64_t)blockIdx.x * seg;
     const uint32_t len = (uint32_t)(min(s + seg, n) - s);
-    const float *xs = x + s;
-    float mn = __builtin_nanf(""), mx = __builtin_nanf("");
+    const V *xs = x + s;
+    V mn = (V)__builtin_nan(""), mx = (V)__builtin_nan("");
     if (VEC) {
-        constexpr uint32_t kStep = 4u * kWG * kRangeUnroll;
+        constexpr uint32_t kStep = kPer * kWG * kRangeUnroll;
         const uint32_t full = len / kStep * kStep;
-        for (uint32_t v = 4u * threadIdx.x; v < full; v += kStep) {
-            float4 q[kRangeUnroll];
+        for (uint32_t v = kPer * threadIdx.x; v < full; v += kStep) {
+            uint4 q[kRangeUnroll];
 #pragma unroll
             for (int u = 0; u < kRangeUnroll; ++u)
-                q[u] = *reinterpret_cast<const float4 *>(xs + v + 4u * kWG * u);
+                q[u] = *reinterpret_cast<const uint4 *>(xs + v + kPer * kWG * u);
 #pragma unroll
-            for (int u = 0; u < kRangeUnroll; ++u) {
-                mn = fminf(mn, fminf(fminf(q[u].x, q[u].y), fminf(q[u].z, q[u].w)));
-                mx = fmaxf(mx, fmaxf(fmaxf(q[u].x, q[u].y), fmaxf(q[u].z, q[u].w)));
-            }
+            for (int u = 0; u < kRangeUnroll; ++u)
+                vec_minmax(reinterpret_cast<const uint8_t *>(&q[u]), mn, mx);
         }
         for (uint32_t i = full + threadIdx.x; i < len; i += kWG) {
-            mn = fminf(mn, xs[i]);
-            mx = fmaxf(mx, xs[i]);
+            mn = vmin(mn, xs[i]);
+            mx = vmax(mx, xs[i]);
         }
     } else {
         for (uint32_t i = threadIdx.x; i < len; i += kWG) {
-            mn = fminf(mn, xs[i]);
-            mx = fmaxf(mx, xs[i]);
+            mn = vmin(mn, xs[i]);
+            mx = vmax(mx, xs[i]);
         }
     }
     range_reduce_wg(mn, mx);
-    if (threadIdx.x == 0) part[blockIdx.x] = RangePart{mn, mx};
+    if (threadIdx.x == 0) part[blockIdx.x] = RangePart<V>{mn, mx};
 }
 
-// eb = fl32(fl32(max - min) * fl32(rel)), NaN when x[0] is NaN
-// (SZp/src/szp.cc:36-47).
-__global__ __launch_bounds__(kWG) void k_range_final(const float *__restrict__ x,
-                                                     const RangePart *part, int nparts,
-                                                     float relf, float *d_eb)
+// eb = (max - min) * rel in the value type (float: fl32(fl32(max - min) *
+// fl32(rel)), SZp/src/szp.cc:36-47; double: SZp/src/szp.cc:70-84), NaN when
+// x[0] is NaN.
+template <typename V>
+__global__ __launch_bounds__(kWG) void k_range_final(const V *__restrict__ x,
+                                                     const RangePart<V> *part, int nparts,
+                                                     V relf, V *d_eb)
 {
-    float mn = __builtin_nanf(""), mx = __builtin_nanf("");
+    V mn = (V)__builtin_nan(""), mx = (V)__builtin_nan("");
     for (int k = threadIdx.x; k < nparts; k += kWG) {
-        const RangePart p = part[k];
-        mn = fminf(mn, p.mn);
-        mx = fmaxf(mx, p.mx);
+        const RangePart<V> p = part[k];
+        mn = vmin(mn, p.mn);
+        mx = vmax(mx, p.mx);
     }
     range_reduce_wg(mn, mx);
     if (threadIdx.x == 0) {
-        float eb;
+        V eb;
         if (isnan(x[0])) {
             eb = x[0];
         } else {
-            const float range = (mx == mn) ? mx - mx : mx - mn;
+            const V range = (mx == mn) ? mx - mx : mx - mn;
             eb = range * relf;
         }
         *d_eb = eb;
@@ -364,16 +428,18 @@ __global__ __launch_bounds__(kWG) void k_range_final(const float *__restrict__ x
 }
 
 // Stream of an empty field: eb and T zero chunk offsets.
-__global__ void k_empty_stream(uint8_t *out, const float *d_eb, float eb_host, uint32_t T,
+template <typename V>
+__global__ void k_empty_stream(uint8_t *out, const V *d_eb, V eb_host, uint32_t T,
                                uint64_t *d_size)
 {
-    const float eb = d_eb ? *d_eb : eb_host;
-    const uint32_t u = __float_as_uint(eb);
-    for (uint32_t i = threadIdx.x; i < 4 + 8 * T; i += blockDim.x) {
-        uint8_t v = i < 4 ? (uint8_t)(u >> (24 - 8 * i)) : 0;
-        out[i] = v;
+    const V eb = d_eb ? *d_eb : eb_host;
+    constexpr uint32_t es = sizeof(V);
+    for (uint32_t i = threadIdx.x; i < es + 8 * T; i += blockDim.x)
+        if (i >= es) out[i] = 0;
+    if (threadIdx.x == 0) {
+        put_eb(out, eb);
+        *d_size = es + 8ull * T;
     }
-    if (threadIdx.x == 0) *d_size = 4 + 8ull * T;
 }
 
 // Unaligned LDS accesses (gfx950 runs with unaligned LDS access enabled; the
@@ -419,29 +485,28 @@ __device__ __forceinline__ void lds_st_bytes(uint8_t *p, uint64_t w, uint32_t n)
 }
 
 // v[0]: the element before the octet (blocks only); v[1..8]: the octet.
+template <typename V>
 struct OctIn {
-    float v[9];
+    V v[9];
 };
 
-__device__ __forceinline__ void octet_load(const float *__restrict__ x, const Oct &s, OctIn &in)
+template <typename V>
+__device__ __forceinline__ void octet_load(const V *__restrict__ x, const Oct &s, OctIn<V> &in)
 {
     if (s.m == 8) {
-        float4 a, b;
-        __builtin_memcpy(&a, x + s.e0, 16);
-        __builtin_memcpy(&b, x + s.e0 + 4, 16);
+        __builtin_memcpy(&in.v[1], x + s.e0, 8 * sizeof(V));
         in.v[0] = x[s.e0 - 1];
-        in.v[1] = a.x; in.v[2] = a.y; in.v[3] = a.z; in.v[4] = a.w;
-        in.v[5] = b.x; in.v[6] = b.y; in.v[7] = b.z; in.v[8] = b.w;
     } else {
-        in.v[0] = (s.m && !s.head) ? x[s.e0 - 1] : 0.0f;
+        in.v[0] = (s.m && !s.head) ? x[s.e0 - 1] : (V)0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) in.v[i + 1] = (uint32_t)i < s.m ? x[s.e0 + i] : 0.0f;
+        for (int i = 0; i < 8; ++i) in.v[i + 1] = (uint32_t)i < s.m ? x[s.e0 + i] : (V)0;
     }
 }
 
 // Lorenzo deltas of the octet: magnitudes, MSB-first sign byte, maximum
 // (SZp/src/szp_float.cc:220-251); the head's raw q for a head slot.
-__device__ __forceinline__ uint32_t octet_deltas(const Oct &s, const OctIn &in, float inv,
+template <typename V>
+__device__ __forceinline__ uint32_t octet_deltas(const Oct &s, const OctIn<V> &in, V inv,
                                                  uint32_t mg[8], uint32_t &sgn, int32_t &qhead)
 {
     sgn = 0;
@@ -477,7 +542,12 @@ __device__ __forceinline__ void encode_octet(uint8_t *ob, uint32_t ib, const Oct
                                              uint32_t b, const uint32_t mg[8], uint32_t sgn,
                                              int32_t qhead)
 {
-    if (!s.m) return;
+    if (!s.m) {
+        // A fastMode-2 unit of one value still writes its block's zero width
+        // byte (SZp/src/szp_float.cc:857-862).
+        if (s.live && !s.head && s.L == 0 && o == 0) ob[ib] = 0;
+        return;
+    }
     if (s.head) {
         lds_st32(ob + ib, (uint32_t)qhead);
         return;
@@ -527,10 +597,10 @@ __device__ __forceinline__ void encode_octet(uint8_t *ob, uint32_t ib, const Oct
 // Full-octet deltas for the simple-tile path: p = element before the octet,
 // a, b = the octet.  mag = max(d, -d) is exact for every int32 d, INT32_MIN
 // included (it maps to 0x80000000 as the reference's unsigned negation does).
-__device__ __forceinline__ uint32_t octet8_deltas(float p, const float4 &a, const float4 &b,
-                                                  float inv, uint32_t mg[8], uint32_t &sgn)
+template <typename V>
+__device__ __forceinline__ uint32_t octet8_deltas(V p, const V v[8], V inv, uint32_t mg[8],
+                                                  uint32_t &sgn)
 {
-    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     int32_t prev = quantize(p, inv);
     uint32_t omax = 0, sg = 0;
 #pragma unroll
@@ -655,6 +725,16 @@ __device__ __forceinline__ float lane_before(float v, float first)
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v),
                                                       0x138, 0xF, 0xF, false));
 }
+__device__ __forceinline__ double lane_before(double v, double first)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v), f = (uint64_t)__double_as_longlong(first);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)u,
+                                                              0x138, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32),
+                                                              (int)(uint32_t)(u >> 32), 0x138, 0xF,
+                                                              0xF, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 // Fast path of k_compress for a simple tile (only full blocks of one chunk)
 // when SPB = B / 8 is a power of two up to 32.  Then K * SPB = 256: round r
@@ -665,9 +745,9 @@ __device__ __forceinline__ float lane_before(float v, float first)
 // offset in the tile; the tile's byte total is published for the look-back
 // before any byte is encoded.  Rounds that fit in ob are encoded there
 // (nbuf of them); the caller writes them out and re-encodes the rest.
-template <int SPB, bool INDEX>
+template <typename V, int SPB, bool INDEX>
 __device__ __forceinline__ void compress_simple_fast(
-    const float *__restrict__ x, float inv, uint64_t Et, uint32_t nti, uint8_t *ob,
+    const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob,
     uint32_t *s_wtot, uint32_t *s_rbase, uint32_t *s_ioff, uint64_t *status, uint64_t tile,
     uint64_t &run, uint32_t &nbuf, uint32_t *idx_carry)
 {
@@ -675,18 +755,17 @@ __device__ __forceinline__ void compress_simple_fast(
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
 
-    float4 va[kMaxRounds], vb[kMaxRounds];
-    float p0[kMaxRounds];
+    V va[kMaxRounds][8];
+    V p0[kMaxRounds];
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
-        va[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-        vb[r] = va[r];
-        p0[r] = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) va[r][i] = (V)0;
+        p0[r] = (V)0;
         if (r * K + d < nti) {
             // Et >= 1 (the tile's first item is a block), so e >= 1.
             const uint64_t e = Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid;
-            __builtin_memcpy(&va[r], x + e, 16);
-            __builtin_memcpy(&vb[r], x + e + 4, 16);
+            __builtin_memcpy(&va[r][0], x + e, 8 * sizeof(V));
             if (lane == 0) p0[r] = x[e - 1];
         }
     }
@@ -697,8 +776,8 @@ __device__ __forceinline__ void compress_simple_fast(
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
         const bool live = r * K + d < nti;
-        const float pv = lane_before(vb[r].w, p0[r]);
-        const uint32_t omax = item_max<SPB>(octet8_deltas(pv, va[r], vb[r], inv, mg[r], sg[r]));
+        const V pv = lane_before(va[r][7], p0[r]);
+        const uint32_t omax = item_max<SPB>(octet8_deltas(pv, va[r], inv, mg[r], sg[r]));
         bw[r] = live ? bit_width(omax) : 0u;
         sz[r] = live ? (bw[r] ? 1u + SPB * (bw[r] + 1u) : 1u) : 0u;
     }
@@ -786,9 +865,9 @@ struct CompressLds {
 
 // One compress tile (all threads of the workgroup call it; every exit is
 // uniform, so a persistent workgroup can go on to its next tile).
-template <bool INDEX, int SPB>
-__device__ void compress_tile(const float *__restrict__ x, const float *__restrict__ d_eb,
-                              float eb_host, const Geom &g, uint8_t *__restrict__ out,
+template <typename V, bool INDEX, int SPB>
+__device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_eb,
+                              V eb_host, const Geom &g, uint8_t *__restrict__ out,
                               uint64_t cap, uint64_t *__restrict__ status,
                               uint32_t *__restrict__ ctl, uint64_t *__restrict__ d_size,
                               uint64_t *__restrict__ idx_pos, uint32_t *__restrict__ idx_carry,
@@ -804,8 +883,10 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
     const int tid = threadIdx.x;
     s_max[tid] = 0;
     if (tid == 0) s_nheads = 0;
-    const float eb = d_eb ? *d_eb : eb_host;
-    const float inv = 1.0f / eb;  // FP32 division, SZp/src/szp_float.cc:185
+    const V eb = d_eb ? *d_eb : eb_host;
+    // 1 / eb in the value type: FP32 division for float (SZp/src/szp_float.cc:185),
+    // FP64 for double (SZp/src/szp_double.cc:185).
+    const V inv = (V)1 / eb;
     const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
     const uint64_t tIN = min(tI0 + (uint64_t)g.K * g.R, g.nitems);
     const uint32_t d = ((uint32_t)tid * g.spb_magic) >> 16;
@@ -828,7 +909,7 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
     bool deferring = false;
     bool published = false;
     if (SPB && simple) {
-        compress_simple_fast<(SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
+        compress_simple_fast<V, (SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
                                                      s_wtot, s_rbase, s_ioff, status, tile, run,
                                                      nbuf, idx_carry);
         published = true;
@@ -837,14 +918,14 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
         // loaded before the first is used, so the tile pays one memory
         // latency and publishes its size about as early as a simple tile.
         Oct oc[kMaxRounds];
-        OctIn oi[kMaxRounds];
+        OctIn<V> oi[kMaxRounds];
         {
             uint32_t t = ct;
             uint64_t j = cj;
 #pragma unroll
             for (int r = 0; r < kMaxRounds; ++r) {
                 const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
-                oc[r] = Oct{0, 0, 0, 0, false, false};
+                oc[r] = Oct{0, 0, 0, 0, false, false, false};
                 if ((uint32_t)r < g.R && ri0 < tIN) {
                     const uint32_t nr = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
                     oc[r] = octet_of(g, t, j, nr, d, o);
@@ -891,7 +972,7 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
             __syncthreads();
             const uint32_t rt = s_tot;
             const uint32_t ib = cur.live ? s_off[d] : 0;
-            if (cur.head && o == 0) {
+            if (cur.first && o == 0) {
                 const uint32_t k = atomicAdd(&s_nheads, 1u);
                 s_head_t[k] = cur.t;
                 s_head_off[k] = (uint32_t)run + ib;
@@ -928,28 +1009,22 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
             uint32_t t = t_first;
             uint64_t j = j_first;
             advance(g, t, j, k);
-            if (j) idx_pos[(uint64_t)t * g.nb0 + (j - 1)] = G0 + s_ioff[k];
+            if (item_indexed(g, j)) idx_pos[(uint64_t)t * g.nb0 + item_block(g, j)] = G0 + s_ioff[k];
         }
     }
     // Chunk offsets relative to the body (SZp/src/szp_float.cc:280-288).
     for (uint32_t k = tid; k < s_nheads; k += kWG) {
         const uint64_t off = s_prefix + s_head_off[k];
-        uint32_t *h = reinterpret_cast<uint32_t *>(out + 4 + 8ull * s_head_t[k]);
+        uint32_t *h = reinterpret_cast<uint32_t *>(out + g.es + 8ull * s_head_t[k]);
         h[0] = (uint32_t)off;
         h[1] = (uint32_t)(off >> 32);
     }
     if (tile == 0) {
-        if (tid == 0) {
-            const uint32_t u = __float_as_uint(eb);
-            out[0] = (uint8_t)(u >> 24);
-            out[1] = (uint8_t)(u >> 16);
-            out[2] = (uint8_t)(u >> 8);
-            out[3] = (uint8_t)u;
-        }
+        if (tid == 0) put_eb(out, eb);
         // Empty chunks (n < T) all start at offset 0.
         if (g.tbs == 0) {
             for (uint32_t k = tid; k + 1 < g.T; k += kWG) {
-                uint32_t *h = reinterpret_cast<uint32_t *>(out + 4 + 8ull * k);
+                uint32_t *h = reinterpret_cast<uint32_t *>(out + g.es + 8ull * k);
                 h[0] = 0;
                 h[1] = 0;
             }
@@ -967,7 +1042,7 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
         uint64_t rj0;
         locate(g, ri0, rt0, rj0);
         const Oct s = octet_of(g, rt0, rj0, nit, d, o);
-        OctIn in;
+        OctIn<V> in;
         octet_load(x, s, in);
         uint32_t mg[8], sgn;
         int32_t qh;
@@ -1014,9 +1089,9 @@ __device__ void compress_tile(const float *__restrict__ x, const float *__restri
     }
 }
 
-template <bool INDEX, int SPB>
-__global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
-                                                  const float *__restrict__ d_eb, float eb_host,
+template <typename V, bool INDEX, int SPB>
+__global__ __launch_bounds__(kWG) void k_compress(const V *__restrict__ x,
+                                                  const V *__restrict__ d_eb, V eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
                                                   uint64_t *__restrict__ status,
                                                   uint32_t *__restrict__ ctl,
@@ -1026,8 +1101,8 @@ __global__ __launch_bounds__(kWG) void k_compress(const float *__restrict__ x,
                                                   TileOrder ord)
 {
     __shared__ CompressLds<INDEX> L;
-    compress_tile<INDEX, SPB>(x, d_eb, eb_host, g, out, cap, status, ctl, d_size, idx_pos,
-                              idx_carry, tile_of_block(ord, blockIdx.x), L);
+    compress_tile<V, INDEX, SPB>(x, d_eb, eb_host, g, out, cap, status, ctl, d_size, idx_pos,
+                                 idx_carry, tile_of_block(ord, blockIdx.x), L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,7 +1117,8 @@ __device__ __forceinline__ uint64_t read_u64_le4(const uint8_t *p)
 
 // Reads the chunk offsets, validates them and lays out the parse tiles:
 // ctab = [cstart[T] | aend[T] | ttab[T+1]] where chunk t's blocks occupy
-// [cstart[t] + 4, aend[t]) and its parse tiles are [ttab[t], ttab[t+1]).
+// [cstart[t] + 4, aend[t]) (mode 2: its units [cstart[t], aend[t])) and its
+// parse tiles are [ttab[t], ttab[t+1]).
 // All workgroups also clear the look-back status words of this call.
 __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, uint32_t S,
                          uint64_t *__restrict__ ctab, uint64_t *__restrict__ clr, uint64_t nclr,
@@ -1057,7 +1133,7 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
     bool bad = len < g.body;
     for (uint32_t t = 0; t < g.T && !bad; ++t) {
         const uint64_t nt = chunk_len(g, t);
-        const uint64_t off = read_u64_le4(st + 4 + 8ull * t);
+        const uint64_t off = read_u64_le4(st + g.es + 8ull * t);
         const uint64_t cs = g.body + off;
         ttab[t] = tiles;
         cstart[t] = cs;
@@ -1066,14 +1142,17 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
             if (off != 0) bad = true;
             continue;
         }
-        const uint64_t ce = (t + 1 < g.T) ? g.body + read_u64_le4(st + 4 + 8ull * (t + 1)) : len;
-        if (off > len || cs + 4 > ce || ce > len) {
+        const uint64_t ce = (t + 1 < g.T) ? g.body + read_u64_le4(st + g.es + 8ull * (t + 1)) : len;
+        // The parsed area: blocks after the chunk's head (mode 1), or all of
+        // the chunk's units (mode 2, at least one of 5 bytes).
+        const uint64_t a0 = g.mode == 2 ? cs : cs + 4;
+        if (off > len || a0 + (g.mode == 2 ? 5 : 0) > ce || a0 > ce || ce > len) {
             bad = true;
             break;
         }
         aend[t] = ce;
         const uint64_t nb = (t == g.T - 1) ? g.nblast : g.nb0;
-        const uint64_t alen = ce - (cs + 4);
+        const uint64_t alen = ce - a0;
         if (nb && !alen) bad = true;
         if (nb) tiles += (alen + S - 1) / S;
     }
@@ -1142,7 +1221,8 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     const uint32_t t = lo;
     const uint64_t first = ttab[t];
     const uint64_t i = gt - first;
-    const uint64_t as = cstart[t] + 4, ae = aend[t];
+    const uint32_t mode = g.mode;
+    const uint64_t as = cstart[t] + (mode == 2 ? 0 : 4), ae = aend[t];
     const uint64_t s = as + i * S;
     const uint64_t te = min(s + S, ae);
     const uint32_t span = (uint32_t)(te - s);
@@ -1150,7 +1230,11 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     const uint32_t sh = (uint32_t)(s - s0);  // buf[p + sh] holds tile byte p
     const uint32_t B = g.B;
 
-    for (uint64_t a = s0 + 16ull * tid; a < te; a += 16ull * kWG) {
+    // The tile's bytes and 16 more (a mode-2 unit starting in the last 4
+    // bytes has its width byte beyond te; the walks' zero runs may look 3
+    // bytes past a sub-tile).
+    const uint64_t tl = min(te + 16, len);
+    for (uint64_t a = s0 + 16ull * tid; a < tl; a += 16ull * kWG) {
         if (a + 16 <= len) {
             *reinterpret_cast<uint4 *>(buf + (a - s0)) = *reinterpret_cast<const uint4 *>(st + a);
         } else {
@@ -1178,12 +1262,12 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
         for (uint32_t c0 = u; c0 < cmax; c0 += 64) {  // wave-uniform loop
             const uint32_t c = c0 + lane;
             uint32_t e = 0;
-            const bool leaves = c < cmax && parse_chain_leaves(tb, c, v, B, e);
+            const bool leaves = c < cmax && parse_chain_leaves(tb, c, v, B, e, mode);
             const uint64_t m = __ballot(leaves);
             if (m) {
                 const int w = __ffsll((long long)m) - 1;  // the lowest candidate that leaves
                 uint32_t n = 0;
-                if (lane == w) n = parse_mark_chain(tb, spec, c, v, B);
+                if (lane == w) n = parse_mark_chain(tb, spec, c, v, B, mode);
                 cnt = __builtin_amdgcn_readlane(n, w);
                 ex = __builtin_amdgcn_readlane(e, w);
                 ok = true;
@@ -1208,7 +1292,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
             const uint32_t cmax = min(ul + W, vl);
             const uint64_t sx = s_exit[lastw];
             for (uint32_t c = ul + tid; c < cmax; c += kWG) {
-                const uint32_t leave = parse_cand_walk(tb, spec, c, vl, B);
+                const uint32_t leave = parse_cand_walk(tb, spec, c, vl, B, mode);
                 if (leave && s + leave != sx) s_nonsync = 1;
             }
         }
@@ -1244,7 +1328,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
             s_broken[w] = broken;
             if (uw >= span || broken || e >= s + vw) continue;  // no block starts here
             uint32_t p = (uint32_t)(e - s), npre = 0;
-            const int res = parse_true_walk(tb, spec, pre, p, vw, B, npre);
+            const int res = parse_true_walk(tb, spec, pre, p, vw, B, npre, mode);
             s_npre[w] = npre;
             if (res == 1) {
                 s_merged[w] = 1;
@@ -1336,12 +1420,22 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
             if (s_broken[w] && wbase[w] + s_npre[w] < nbt) bad = true;
         if (gt + 1 == ttab[t + 1] && prefix + tile_cnt < nbt) bad = true;  // truncated chunk
         if (s_lastpos != ~0ull) {
-            // The last block is shorter when (n_t - 1) % B != 0; its end must
-            // be the chunk end (exact for all but the last chunk).
+            // The last block is shorter when (n_t - 1) % B != 0 (mode 2: its
+            // unit when n_t % B != 0); its end must be the chunk end (exact
+            // for all but the last chunk).
             const uint64_t pl = s_lastpos;
-            const uint32_t b = st[pl];
-            const uint64_t Ll = chunk_len(g, t) - 1 - (nbt - 1) * (uint64_t)g.B;
-            const uint64_t end = pl + block_bytes(b, (uint32_t)Ll);
+            uint64_t end;
+            uint32_t b;
+            if (mode == 2) {
+                const uint64_t Ll = chunk_len(g, t) - (nbt - 1) * (uint64_t)g.B - 1;
+                b = pl + 4 < len ? st[pl + 4] : 255u;
+                end = pl + 4 + block_bytes(b, (uint32_t)Ll);
+                if (!Ll && pl + 4 < len) b = 0;  // a 1-value unit's width byte is not read
+            } else {
+                const uint64_t Ll = chunk_len(g, t) - 1 - (nbt - 1) * (uint64_t)g.B;
+                b = st[pl];
+                end = pl + block_bytes(b, (uint32_t)Ll);
+            }
             if (b > 32 || (t + 1 < g.T ? end != ae : end > len)) bad = true;
             if (t + 1 == g.T) ctl[4] = (uint32_t)end, ctl[5] = (uint32_t)(end >> 32);
         }
@@ -1501,12 +1595,12 @@ __device__ __forceinline__ void load_bytes(const uint8_t *__restrict__ st, uint6
 // the LDS buffer are read one round at a time.  Returns false, before any
 // side effect, for the field's last tile (its end is not in the index); the
 // caller then takes the general path.
-template <int SPB, bool INDEXED>
+template <typename V, int SPB, bool INDEXED>
 __device__ __forceinline__ bool decode_simple_fast(
     const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
     const uint64_t *__restrict__ blkpos, const uint32_t *__restrict__ idx_carry,
-    float *__restrict__ out, uint64_t *status, uint32_t *ctl, uint64_t tile, uint64_t tIN,
-    uint32_t ct, uint64_t cj, uint32_t nti, float eb, uint8_t *buf, uint32_t *s_wsum,
+    V *__restrict__ out, uint64_t *status, uint32_t *ctl, uint64_t tile, uint64_t tIN,
+    uint32_t ct, uint64_t cj, uint32_t nti, V eb, uint8_t *buf, uint32_t *s_wsum,
     uint64_t *s_lb, int *s_bad)
 {
     constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
@@ -1523,7 +1617,7 @@ __device__ __forceinline__ bool decode_simple_fast(
     uint64_t j1;
     locate(g, tIN, t1, j1);
     const uint64_t P1 = j1 ? blkpos[(uint64_t)t1 * g.nb0 + (j1 - 1)]
-                           : g.body + read_u64_le4(st + 4 + 8ull * t1);
+                           : g.body + read_u64_le4(st + g.es + 8ull * t1);
     if (tid == 0) *s_bad = 0;
     bool bad = !(P0 < P1 && P1 <= len);
     uint32_t dl[kMaxRounds][8];
@@ -1611,18 +1705,10 @@ __device__ __forceinline__ bool decode_simple_fast(
         for (int w = 0; w < 4; ++w) base += s_wsum[r * 4 + w];
         if (r * K + d < nti) {
             const uint32_t q0 = before + ex[r];
-            float4 a, c;
-            a.x = (float)(int32_t)(q0 + dl[r][0]) * eb;
-            a.y = (float)(int32_t)(q0 + dl[r][1]) * eb;
-            a.z = (float)(int32_t)(q0 + dl[r][2]) * eb;
-            a.w = (float)(int32_t)(q0 + dl[r][3]) * eb;
-            c.x = (float)(int32_t)(q0 + dl[r][4]) * eb;
-            c.y = (float)(int32_t)(q0 + dl[r][5]) * eb;
-            c.z = (float)(int32_t)(q0 + dl[r][6]) * eb;
-            c.w = (float)(int32_t)(q0 + dl[r][7]) * eb;
-            float *dst = out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid;
-            __builtin_memcpy(dst, &a, 16);
-            __builtin_memcpy(dst + 4, &c, 16);
+            V y[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
+            __builtin_memcpy(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y, 8 * sizeof(V));
         }
     }
     return true;
@@ -1651,10 +1737,10 @@ struct DecodeLds {
 
 // One decode tile (all threads of the workgroup call it; every exit is
 // uniform).
-template <bool INDEXED, int SPB>
+template <typename V, bool INDEXED, int SPB>
 __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
                             const uint64_t *__restrict__ blkpos,
-                            const uint32_t *__restrict__ idx_carry, float *__restrict__ out,
+                            const uint32_t *__restrict__ idx_carry, V *__restrict__ out,
                             uint64_t *__restrict__ status, uint32_t *__restrict__ ctl,
                             uint64_t tile, DecodeLds &L)
 {
@@ -1665,9 +1751,7 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
     int &s_bad = L.s_bad;
 
     const int tid = threadIdx.x, lane = __lane_id();
-    const uint32_t ebu = ((uint32_t)st[0] << 24) | ((uint32_t)st[1] << 16) |
-                         ((uint32_t)st[2] << 8) | (uint32_t)st[3];
-    const float eb = __uint_as_float(ebu);
+    const V eb = get_eb<V>(st);
     const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
     const uint64_t tIN = min(tI0 + (uint64_t)g.K * g.R, g.nitems);
     const uint32_t nti = (uint32_t)(tIN - tI0);
@@ -1675,7 +1759,7 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
     uint64_t j0;
     locate(g, tI0, t0, j0);
     if (SPB && simple_tile(g, t0, j0, nti) &&
-        decode_simple_fast<(SPB ? SPB : 1), INDEXED>(st, len, g, blkpos, idx_carry, out, status,
+        decode_simple_fast<V, (SPB ? SPB : 1), INDEXED>(st, len, g, blkpos, idx_carry, out, status,
                                                      ctl, tile, tIN, t0, j0, nti, eb, buf,
                                                      L.s_wsum, L.s_lb, &s_bad))
         return;
@@ -1690,13 +1774,17 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
             uint32_t t = t0;
             uint64_t j = j0;
             advance(g, t, j, it);
-            // Chunk heads sit at the offsets of the stream header.
-            mypos[k] = j == 0 ? g.body + read_u64_le4(st + 4 + 8ull * t)
-                              : blkpos[(uint64_t)t * g.nb0 + (j - 1)];
+            // Chunk heads sit at the offsets of the stream header; a fastMode-2
+            // block follows its unit's 4-byte head.
+            if (g.mode == 2)
+                mypos[k] = blkpos[(uint64_t)t * g.nb0 + (j >> 1)] + ((j & 1) ? 4u : 0u);
+            else
+                mypos[k] = j == 0 ? g.body + read_u64_le4(st + g.es + 8ull * t)
+                                  : blkpos[(uint64_t)t * g.nb0 + (j - 1)];
             if (it == 0) s_base = mypos[k];
             if (it == nti - 1) {
                 uint64_t e;
-                if (j == 0) {
+                if (item_head(g, j)) {
                     e = mypos[k] + 4;
                 } else {
                     const uint32_t b = mypos[k] < len ? st[mypos[k]] : 255u;
@@ -1765,7 +1853,9 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
             }
             uint32_t dl[8], h = 0, m = 0;
             bool b2 = false;
-            if (active) m = decode_slot(L, j == 0, o, buf, s_pos[rfirst + d] - rb, rend - rb, dl, h, b2);
+            if (active)
+                m = decode_slot(L, item_head(g, j), o, buf, s_pos[rfirst + d] - rb, rend - rb, dl, h,
+                                b2);
             if (b2) s_bad = 1;
             const uint32_t v = m ? (h ? dl[0] : dl[m - 1]) : 0u;
             uint32_t eh, ev, rh, rv;
@@ -1774,25 +1864,18 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
                 // carry (+) tile prefix of the round (+) slot prefix in the round
                 uint32_t before = th ? tv : carry + tv;
                 before = eh ? ev : before + ev;
-                float *dst = out + es + (h ? 0u : 8u * o);
+                V *dst = out + es + (h ? 0u : 8u * o);
                 if (h) {
-                    dst[0] = (float)(int32_t)dl[0] * eb;
+                    dst[0] = (V)(int32_t)dl[0] * eb;
                 } else if (m == 8) {
-                    float4 a, c;
-                    a.x = (float)(int32_t)(before + dl[0]) * eb;
-                    a.y = (float)(int32_t)(before + dl[1]) * eb;
-                    a.z = (float)(int32_t)(before + dl[2]) * eb;
-                    a.w = (float)(int32_t)(before + dl[3]) * eb;
-                    c.x = (float)(int32_t)(before + dl[4]) * eb;
-                    c.y = (float)(int32_t)(before + dl[5]) * eb;
-                    c.z = (float)(int32_t)(before + dl[6]) * eb;
-                    c.w = (float)(int32_t)(before + dl[7]) * eb;
-                    __builtin_memcpy(dst, &a, 16);
-                    __builtin_memcpy(dst + 4, &c, 16);
+                    V y[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(before + dl[i]) * eb;
+                    __builtin_memcpy(dst, y, 8 * sizeof(V));
                 } else {
 #pragma unroll
                     for (int i = 0; i < 8; ++i)
-                        if ((uint32_t)i < m) dst[i] = (float)(int32_t)(before + dl[i]) * eb;
+                        if ((uint32_t)i < m) dst[i] = (V)(int32_t)(before + dl[i]) * eb;
                 }
             }
             tv = rh ? rv : tv + rv;
@@ -1805,7 +1888,7 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
         if (tid < 64) {
             if (th && lane == 0) st_status(&status[tile], kFlagIncl | tv);
             uint32_t c = 0;
-            if (tile > 0 && j0 != 0) {
+            if (tile > 0 && !item_head(g, j0)) {
                 if (!th && lane == 0) st_status(&status[tile], kFlagAgg | tv);
                 int64_t hi = (int64_t)tile - 1;
                 while (true) {
@@ -1836,11 +1919,11 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
     }
 }
 
-template <bool INDEXED, int SPB>
+template <typename V, bool INDEXED, int SPB>
 __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, uint64_t len,
                                                 Geom g, const uint64_t *__restrict__ blkpos,
                                                 const uint32_t *__restrict__ idx_carry,
-                                                float *__restrict__ out,
+                                                V *__restrict__ out,
                                                 uint64_t *__restrict__ status,
                                                 uint32_t *__restrict__ ctl, TileOrder ord)
 {
@@ -1851,20 +1934,55 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
         if (!INDEXED && threadIdx.x == 0) st_status(&status[tile], kFlagIncl);
         return;
     }
-    decode_tile<INDEXED, SPB>(st, len, g, blkpos, idx_carry, out, status, ctl, tile, L);
+    decode_tile<V, INDEXED, SPB>(st, len, g, blkpos, idx_carry, out, status, ctl, tile, L);
+}
+
+// fastMode 2 with B = 1: every unit is one value, which the compressor
+// writes as [q i32][0] (SZp/src/szp_float.cc:817-862) while the reference's
+// decompressor reads 4 bytes per unit and never the zero width byte
+// (SZp/src/szpd_float.cc:472-483, the `current_block_size > 1` test), so
+// value k of chunk t comes from the 4 bytes at the chunk's offset + 4k.  This
+// kernel reproduces that decoding exactly, as a drop-in must.
+template <typename V>
+__global__ __launch_bounds__(kWG) void k_decode_ra_unit1(const uint8_t *__restrict__ st,
+                                                         uint64_t len, Geom g,
+                                                         V *__restrict__ out,
+                                                         uint32_t *__restrict__ ctl)
+{
+    const V eb = get_eb<V>(st);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
+        const uint32_t t = g.tbs ? (uint32_t)min<uint64_t>(i / g.tbs, g.T - 1) : g.T - 1;
+        const uint64_t p = g.body + read_u64_le4(st + g.es + 8ull * t) + 4 * (i - chunk_lo(g, t));
+        if (p + 4 > len) {
+            raise_error(ctl, kErrStream);
+            return;
+        }
+        const uint32_t q = (uint32_t)st[p] | ((uint32_t)st[p + 1] << 8) |
+                           ((uint32_t)st[p + 2] << 16) | ((uint32_t)st[p + 3] << 24);
+        out[i] = (V)(int32_t)q * eb;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
-uint64_t compress_bound(uint64_t n, int B, int T)
+uint64_t compress_bound(uint64_t n, int B, int T, int dtype, int mode)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g)) return 0;
-    const uint64_t per_block = 1 + ((uint64_t)B + 7) / 8;
+    if (!make_geom(n, B, T, &g, dtype == 1 ? 8u : 4u, (uint32_t)mode)) return 0;
     uint64_t total = g.body;
     const uint64_t nl = n - (uint64_t)(T - 1) * g.tbs;
+    if (mode == 2) {
+        // Per unit: 4 head bytes, a width byte and ceil((B - 1) / 8) sign
+        // bytes at most; 4 bytes per other value.
+        const uint64_t per_unit = 5 + ((uint64_t)B - 1 + 7) / 8;
+        if (g.tbs) total += (uint64_t)(T - 1) * (g.nb0 * per_unit + 4 * (g.tbs - g.nb0));
+        if (nl) total += g.nblast * per_unit + 4 * (nl - g.nblast);
+        return total;
+    }
+    const uint64_t per_block = 1 + ((uint64_t)B + 7) / 8;
     if (g.tbs) total += (uint64_t)(T - 1) * (4 + g.nb0 * per_block + 4 * (g.tbs - 1));
     if (nl) total += 4 + g.nblast * per_block + 4 * (nl - 1);
     return total;
@@ -1981,7 +2099,7 @@ constexpr uint64_t kBatchSeg = 65536;  // elements per range segment
 // Min/max of one segment of a REL field (index-free, as k_range_partial).
 __global__ __launch_bounds__(kWG) void k_batch_range_partial(const BatchJob *__restrict__ jobs,
                                                              const uint32_t *__restrict__ smap,
-                                                             RangePart *__restrict__ parts)
+                                                             RangePart<float> *__restrict__ parts)
 {
     const uint32_t sg = blockIdx.x;
     const BatchJob &J = jobs[smap[sg]];
@@ -2008,18 +2126,18 @@ __global__ __launch_bounds__(kWG) void k_batch_range_partial(const BatchJob *__r
         }
     }
     range_reduce_wg(mn, mx);
-    if (threadIdx.x == 0) parts[sg] = RangePart{mn, mx};
+    if (threadIdx.x == 0) parts[sg] = RangePart<float>{mn, mx};
 }
 
 // eb of every REL field from its segments (one workgroup per field).
 __global__ __launch_bounds__(kWG) void k_batch_range_final(BatchJob *__restrict__ jobs,
-                                                           const RangePart *__restrict__ parts)
+                                                           const RangePart<float> *__restrict__ parts)
 {
     BatchJob &J = jobs[blockIdx.x];
     if (!J.rel_mode) return;
     float mn = __builtin_nanf(""), mx = __builtin_nanf("");
     for (uint32_t k = threadIdx.x; k < J.nseg; k += kWG) {
-        const RangePart p = parts[J.seg0 + k];
+        const RangePart<float> p = parts[J.seg0 + k];
         mn = fminf(mn, p.mn);
         mx = fmaxf(mx, p.mx);
     }
@@ -2060,8 +2178,8 @@ __global__ __launch_bounds__(kWG) void k_batch_compress(const BatchJob *__restri
             const QEntry e = qmap[q];
             const BatchJob &J = jobs[e.job];
             const Geom g = J.g;
-            compress_tile<INDEX, SPB>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl, J.d_size,
-                                      J.idx_pos, J.idx_carry, e.tile, L);
+            compress_tile<float, INDEX, SPB>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl,
+                                             J.d_size, J.idx_pos, J.idx_carry, e.tile, L);
             __syncthreads();
         }
     }
@@ -2088,8 +2206,8 @@ __global__ __launch_bounds__(kWG) void k_batch_decode(const BatchJob *__restrict
             const QEntry e = qmap[q];
             const BatchJob &J = jobs[e.job];
             const Geom g = J.g;
-            decode_tile<true, SPB>(J.st, J.cap, g, J.idx_pos, J.idx_carry, J.y, nullptr, ctl,
-                                   e.tile, L);
+            decode_tile<float, true, SPB>(J.st, J.cap, g, J.idx_pos, J.idx_carry, J.y, nullptr,
+                                          ctl, e.tile, L);
             __syncthreads();
         }
     }
@@ -2139,8 +2257,8 @@ uint64_t align_up(uint64_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct CompressWs {
     uint32_t *ctl;
-    float *eb;
-    RangePart *parts;
+    void *eb;     // the REL bound, float or double
+    void *parts;  // RangePart<V>[kRangeParts]
     uint64_t *status;
 };
 
@@ -2149,10 +2267,10 @@ uint64_t compress_ws_layout(const Geom &g, uint8_t *base, CompressWs *w)
     uint64_t off = 0;
     if (w) w->ctl = reinterpret_cast<uint32_t *>(base + off);
     off += align_up(64);
-    if (w) w->eb = reinterpret_cast<float *>(base + off);
+    if (w) w->eb = base + off;
     off += align_up(16);
-    if (w) w->parts = reinterpret_cast<RangePart *>(base + off);
-    off += align_up(sizeof(RangePart) * kRangeParts);
+    if (w) w->parts = base + off;
+    off += align_up(sizeof(RangePart<double>) * kRangeParts);
     if (w) w->status = reinterpret_cast<uint64_t *>(base + off);
     off += align_up(8 * (g.ntiles ? g.ntiles : 1));
     return off;
@@ -2168,26 +2286,33 @@ struct DecompressWs {
     uint64_t nclr;
 };
 
-// Sub-tile bytes of the parse: at least the longest block, so that a
-// sub-tile's entry always lies inside it.
-uint32_t parse_subtile_bytes(int B)
+// Longest step of the parse: a block of B values at width 32 (mode 2: a
+// unit, its 4-byte head and a block of B - 1 values).
+uint32_t parse_max_step(int B, int mode)
 {
-    const uint32_t W = block_bytes(32, (uint32_t)B);
+    return mode == 2 ? 4 + block_bytes(32, (uint32_t)B - 1) : block_bytes(32, (uint32_t)B);
+}
+
+// Sub-tile bytes of the parse: at least the longest step, so that a
+// sub-tile's entry always lies inside it.
+uint32_t parse_subtile_bytes(int B, int mode)
+{
+    const uint32_t W = parse_max_step(B, mode);
     const uint32_t U = W > 4096 ? W : 4096;
     return (U + 255) & ~255u;
 }
 
-uint32_t parse_tile_bytes(int B) { return 4 * parse_subtile_bytes(B); }
+uint32_t parse_tile_bytes(int B, int mode) { return 4 * parse_subtile_bytes(B, mode); }
 
-size_t parse_lds_bytes(int B)
+size_t parse_lds_bytes(int B, int mode)
 {
-    const uint32_t S = parse_tile_bytes(B), NW = S / 32;
+    const uint32_t S = parse_tile_bytes(B, mode), NW = S / 32;
     return ((S + 64 + 15) & ~15u) + 2ull * NW * 4 + 2ull * NW * 2;
 }
 
 uint64_t decompress_ws_layout(const Geom &g, uint64_t len, uint8_t *base, DecompressWs *w)
 {
-    const uint32_t S = parse_tile_bytes((int)g.B);
+    const uint32_t S = parse_tile_bytes((int)g.B, (int)g.mode);
     const uint64_t nt1 = len / S + 1 + g.T;
     const uint64_t nblocks = (uint64_t)(g.T - 1) * g.nb0 + g.nblast;
     uint64_t off = 0;
@@ -2211,10 +2336,10 @@ uint64_t decompress_ws_layout(const Geom &g, uint64_t len, uint8_t *base, Decomp
 
 }  // namespace
 
-uint64_t index_bytes(uint64_t n, int B, int T)
+uint64_t index_bytes(uint64_t n, int B, int T, int mode)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g)) return 0;
+    if (!make_geom(n, B, T, &g, 4, (uint32_t)mode)) return 0;
     const uint64_t nblocks = (uint64_t)(g.T - 1) * g.nb0 + g.nblast;
     return align_up(8 * (nblocks ? nblocks : 1)) + align_up(4 * (g.ntiles ? g.ntiles : 1));
 }
@@ -2226,17 +2351,17 @@ static void index_layout(const Geom &g, uint8_t *base, uint64_t **pos, uint32_t 
     *carry = reinterpret_cast<uint32_t *>(base + align_up(8 * (nblocks ? nblocks : 1)));
 }
 
-uint64_t compress_workspace_bytes(uint64_t n, int B, int T)
+uint64_t compress_workspace_bytes(uint64_t n, int B, int T, int mode)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g)) return 0;
+    if (!make_geom(n, B, T, &g, 4, (uint32_t)mode)) return 0;
     return compress_ws_layout(g, nullptr, nullptr);
 }
 
-uint64_t decompress_workspace_bytes(uint64_t stream_bytes, uint64_t n, int B, int T)
+uint64_t decompress_workspace_bytes(uint64_t stream_bytes, uint64_t n, int B, int T, int mode)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g)) return 0;
+    if (!make_geom(n, B, T, &g, 4, (uint32_t)mode)) return 0;
     return decompress_ws_layout(g, stream_bytes, nullptr, nullptr);
 }
 
@@ -2250,9 +2375,9 @@ enum { kOk = 0, kArg = 1, kCap = 2, kHip = 3 };
 
 // k_compress with the fast simple-tile path compiled for SPB = B / 8 when
 // that is a power of two up to 32 (B = 8 ... 256), else without it.
-template <bool INDEX>
-static void launch_k_compress(const Geom &g, hipStream_t stream, const float *d_x, const float *d_eb,
-                              float eb_host, uint8_t *d_out, uint64_t cap, uint64_t *status,
+template <typename V, bool INDEX>
+static void launch_k_compress(const Geom &g, hipStream_t stream, const V *d_x, const V *d_eb,
+                              V eb_host, uint8_t *d_out, uint64_t cap, uint64_t *status,
                               uint32_t *ctl, uint64_t *d_size, uint64_t *ipos, uint32_t *icarry)
 {
     const dim3 grid((uint32_t)g.ntiles), block(kWG);
@@ -2261,7 +2386,7 @@ static void launch_k_compress(const Geom &g, hipStream_t stream, const float *d_
     switch (spb) {
 #define SZP_COMPRESS_CASE(S)                                                                     \
     case S:                                                                                      \
-        SZP_LAUNCH("k_compress", (k_compress<INDEX, S>), grid, block, 0, stream, d_x, d_eb,      \
+        SZP_LAUNCH("k_compress", (k_compress<V, INDEX, S>), grid, block, 0, stream, d_x, d_eb,   \
                    eb_host, g, d_out, cap, status, ctl, d_size, ipos, icarry, ord);              \
         break;
         SZP_COMPRESS_CASE(1)
@@ -2272,17 +2397,17 @@ static void launch_k_compress(const Geom &g, hipStream_t stream, const float *d_
         SZP_COMPRESS_CASE(32)
 #undef SZP_COMPRESS_CASE
     default:
-        SZP_LAUNCH("k_compress", (k_compress<INDEX, 0>), grid, block, 0, stream, d_x, d_eb, eb_host,
+        SZP_LAUNCH("k_compress", (k_compress<V, INDEX, 0>), grid, block, 0, stream, d_x, d_eb, eb_host,
                    g, d_out, cap, status, ctl, d_size, ipos, icarry, ord);
     }
 }
 
 // k_decode with the fast simple-tile path compiled for SPB = B / 8 when that
 // is a power of two up to 32.
-template <bool INDEXED>
+template <typename V, bool INDEXED>
 static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
                             const uint8_t *d_stream, uint64_t len, const uint64_t *blkpos,
-                            const uint32_t *carry, float *d_out, uint64_t *status, uint32_t *ctl)
+                            const uint32_t *carry, V *d_out, uint64_t *status, uint32_t *ctl)
 {
     const dim3 grid((uint32_t)g.ntiles), block(kWG);
     const uint32_t spb = (g.B & 7) ? 0u : g.B / 8;
@@ -2294,7 +2419,7 @@ static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
     switch (spb) {
 #define SZP_DECODE_CASE(S)                                                                       \
     case S:                                                                                      \
-        SZP_LAUNCH(name, (k_decode<INDEXED, S>), grid, block, 0, stream, d_stream, len, g,       \
+        SZP_LAUNCH(name, (k_decode<V, INDEXED, S>), grid, block, 0, stream, d_stream, len, g,    \
                    blkpos, carry, d_out, status, ctl, ord);                                      \
         break;
         SZP_DECODE_CASE(1)
@@ -2305,109 +2430,184 @@ static void launch_k_decode(const char *name, const Geom &g, hipStream_t stream,
         SZP_DECODE_CASE(32)
 #undef SZP_DECODE_CASE
     default:
-        SZP_LAUNCH(name, (k_decode<INDEXED, 0>), grid, block, 0, stream, d_stream, len, g, blkpos,
+        SZP_LAUNCH(name, (k_decode<V, INDEXED, 0>), grid, block, 0, stream, d_stream, len, g, blkpos,
                    carry, d_out, status, ctl, ord);
     }
 }
 
-int launch_compress(const float *d_x, uint64_t n, int err_mode, double abs_eb, double rel, int B,
-                    int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size, void *d_work,
-                    uint64_t work_bytes, hipStream_t stream, void *d_index, uint64_t idx_bytes)
+template <typename V>
+static int launch_compress_t(int mode, const V *d_x, uint64_t n, int err_mode, double abs_eb,
+                             double rel, int B, int T, uint8_t *d_out, uint64_t cap,
+                             uint64_t *d_size, void *d_work, uint64_t work_bytes,
+                             hipStream_t stream, void *d_index, uint64_t idx_bytes)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g) || !d_out || !d_size || !d_work) return kArg;
+    if (!make_geom(n, B, T, &g, sizeof(V), (uint32_t)mode) || !d_out || !d_size || !d_work)
+        return kArg;
     if (err_mode != 0 && err_mode != 1) return kArg;
     if (err_mode == 1 && n == 0) return kArg;
     if (n && !d_x) return kArg;
     if ((reinterpret_cast<uintptr_t>(d_out) & 15) || (reinterpret_cast<uintptr_t>(d_work) & 255))
         return kArg;
     if (cap < g.body) return kCap;
-    if (d_index && ((reinterpret_cast<uintptr_t>(d_index) & 255) || idx_bytes < index_bytes(n, B, T)))
-        return d_index && idx_bytes < index_bytes(n, B, T) ? kCap : kArg;
+    if (d_index && ((reinterpret_cast<uintptr_t>(d_index) & 255) ||
+                    idx_bytes < index_bytes(n, B, T, mode)))
+        return d_index && idx_bytes < index_bytes(n, B, T, mode) ? kCap : kArg;
     CompressWs w;
     if (compress_ws_layout(g, reinterpret_cast<uint8_t *>(d_work), &w) > work_bytes) return kCap;
 
     const uint64_t nst = g.ntiles ? g.ntiles : 1;
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((nst + kWG - 1) / kWG, 4096);
     SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
-    const float *d_eb = nullptr;
-    const float eb_host = (float)abs_eb;  // SZp/src/szp.cc:32
+    const V *d_eb = nullptr;
+    // ABS: cast to the value type (float: SZp/src/szp.cc:32; double: :62).
+    const V eb_host = (V)abs_eb;
     const bool vec = (reinterpret_cast<uintptr_t>(d_x) & 15) == 0;
     if (err_mode == 1) {
         uint64_t seg = (n + kRangeParts - 1) / kRangeParts;
         seg = (seg + 4095) & ~4095ull;  // a multiple of 4 * kWG * kRangeUnroll
         const int np = (int)((n + seg - 1) / seg);
+        RangePart<V> *parts = reinterpret_cast<RangePart<V> *>(w.parts);
+        V *d_ebw = reinterpret_cast<V *>(w.eb);
         if (vec)
-            SZP_LAUNCH("k_range_partial", k_range_partial<true>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);
+            SZP_LAUNCH("k_range_partial", (k_range_partial<V, true>), dim3(np), dim3(kWG), 0, stream, d_x, n, seg, parts);
         else
-            SZP_LAUNCH("k_range_partial", k_range_partial<false>, dim3(np), dim3(kWG), 0, stream, d_x, n, seg, w.parts);
-        SZP_LAUNCH("k_range_final", k_range_final, dim3(1), dim3(kWG), 0, stream, d_x, w.parts, np,
-                           (float)rel, w.eb);
-        d_eb = w.eb;
+            SZP_LAUNCH("k_range_partial", (k_range_partial<V, false>), dim3(np), dim3(kWG), 0, stream, d_x, n, seg, parts);
+        SZP_LAUNCH("k_range_final", k_range_final<V>, dim3(1), dim3(kWG), 0, stream, d_x, parts, np,
+                   (V)rel, d_ebw);
+        d_eb = d_ebw;
     }
     if (n == 0) {
-        SZP_LAUNCH("k_empty_stream", k_empty_stream, dim3(1), dim3(kWG), 0, stream, d_out, d_eb, eb_host,
+        SZP_LAUNCH("k_empty_stream", k_empty_stream<V>, dim3(1), dim3(kWG), 0, stream, d_out, d_eb, eb_host,
                            g.T, d_size);
     } else if (d_index) {
         uint64_t *ipos;
         uint32_t *icarry;
         index_layout(g, reinterpret_cast<uint8_t *>(d_index), &ipos, &icarry);
-        launch_k_compress<true>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl, d_size,
-                                ipos, icarry);
+        launch_k_compress<V, true>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl,
+                                   d_size, ipos, icarry);
     } else {
-        launch_k_compress<false>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl, d_size,
-                                 nullptr, nullptr);
+        launch_k_compress<V, false>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl,
+                                    d_size, nullptr, nullptr);
     }
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 
-int launch_decompress(const uint8_t *d_stream, uint64_t len, uint64_t n, int B, int T,
-                      float *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream)
+int launch_compress(int dtype, const void *d_x, uint64_t n, int err_mode, double abs_eb,
+                    double rel, int B, int T, uint8_t *d_out, uint64_t cap, uint64_t *d_size,
+                    void *d_work, uint64_t work_bytes, hipStream_t stream, void *d_index,
+                    uint64_t idx_bytes, int mode)
+{
+    if (dtype == 0)
+        return launch_compress_t<float>(mode, reinterpret_cast<const float *>(d_x), n, err_mode,
+                                        abs_eb, rel, B, T, d_out, cap, d_size, d_work, work_bytes,
+                                        stream, d_index, idx_bytes);
+    if (dtype == 1)
+        return launch_compress_t<double>(mode, reinterpret_cast<const double *>(d_x), n, err_mode,
+                                         abs_eb, rel, B, T, d_out, cap, d_size, d_work,
+                                         work_bytes, stream, d_index, idx_bytes);
+    return kArg;
+}
+
+template <typename V>
+static int launch_decompress_t(int mode, const uint8_t *d_stream, uint64_t len, uint64_t n, int B,
+                               int T, V *d_out, void *d_work, uint64_t work_bytes,
+                               hipStream_t stream)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g) || !d_stream || !d_work) return kArg;
+    if (!make_geom(n, B, T, &g, sizeof(V), (uint32_t)mode) || !d_stream || !d_work) return kArg;
     if (n && !d_out) return kArg;
     if ((reinterpret_cast<uintptr_t>(d_stream) & 15) || (reinterpret_cast<uintptr_t>(d_work) & 255))
         return kArg;
     DecompressWs w;
     if (decompress_ws_layout(g, len, reinterpret_cast<uint8_t *>(d_work), &w) > work_bytes)
         return kCap;
-    const uint32_t S = parse_tile_bytes(B);
-    const uint32_t W = block_bytes(32, (uint32_t)B);
+    if (mode == 2 && B == 1) {
+        // Units of one value decode as the reference does (k_decode_ra_unit1).
+        if (len < g.body) return kArg;
+        SZP_LAUNCH("k_clear", k_clear, dim3(1), dim3(kWG), 0, stream, w.ctl, nullptr, 0);
+        if (n)
+            SZP_LAUNCH("k_decode_ra_unit1", k_decode_ra_unit1<V>,
+                       dim3((uint32_t)std::min<uint64_t>((n + kWG - 1) / kWG, 8192)), dim3(kWG), 0,
+                       stream, d_stream, len, g, d_out, w.ctl);
+        return hipGetLastError() == hipSuccess ? kOk : kHip;
+    }
+    const uint32_t S = parse_tile_bytes(B, mode);
+    const uint32_t W = parse_max_step(B, mode);
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((w.nclr + kWG - 1) / kWG, 4096);
     SZP_LAUNCH("k_dsetup", k_dsetup, dim3(cgrid ? cgrid : 1), dim3(kWG), 0, stream, d_stream, len, g,
                        S, w.ctab, w.exit_st, w.nclr, w.ctl);
     if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;
     const uint64_t nt1 = len / S + 1 + g.T;
-    SZP_LAUNCH("k_parse", k_parse, dim3((uint32_t)nt1), dim3(kWG), parse_lds_bytes(B), stream,
-               d_stream, len, g, parse_subtile_bytes(B), W, w.ctab, w.exit_st, w.cnt_st, w.blkpos,
+    SZP_LAUNCH("k_parse", k_parse, dim3((uint32_t)nt1), dim3(kWG), parse_lds_bytes(B, mode), stream,
+               d_stream, len, g, parse_subtile_bytes(B, mode), W, w.ctab, w.exit_st, w.cnt_st, w.blkpos,
                w.ctl);
-    launch_k_decode<false>("k_decode", g, stream, d_stream, len, w.blkpos, nullptr, d_out,
-                           w.dec_st, w.ctl);
+    launch_k_decode<V, false>("k_decode", g, stream, d_stream, len, w.blkpos, nullptr, d_out,
+                              w.dec_st, w.ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 
-int launch_decompress_indexed(const uint8_t *d_stream, uint64_t len, const void *d_index,
-                              uint64_t idx_bytes, uint64_t n, int B, int T, float *d_out,
-                              void *d_work, uint64_t work_bytes, hipStream_t stream)
+int launch_decompress(int dtype, const uint8_t *d_stream, uint64_t len, uint64_t n, int B, int T,
+                      void *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream, int mode)
+{
+    if (dtype == 0)
+        return launch_decompress_t<float>(mode, d_stream, len, n, B, T,
+                                          reinterpret_cast<float *>(d_out), d_work, work_bytes,
+                                          stream);
+    if (dtype == 1)
+        return launch_decompress_t<double>(mode, d_stream, len, n, B, T,
+                                           reinterpret_cast<double *>(d_out), d_work, work_bytes,
+                                           stream);
+    return kArg;
+}
+
+template <typename V>
+static int launch_decompress_indexed_t(int mode, const uint8_t *d_stream, uint64_t len,
+                                       const void *d_index, uint64_t idx_bytes, uint64_t n, int B,
+                                       int T, V *d_out, void *d_work, uint64_t work_bytes,
+                                       hipStream_t stream)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g) || !d_stream || !d_work || !d_index) return kArg;
+    if (!make_geom(n, B, T, &g, sizeof(V), (uint32_t)mode) || !d_stream || !d_work || !d_index)
+        return kArg;
     if (n && !d_out) return kArg;
     if ((reinterpret_cast<uintptr_t>(d_stream) & 15) || (reinterpret_cast<uintptr_t>(d_work) & 255) ||
         (reinterpret_cast<uintptr_t>(d_index) & 255))
         return kArg;
-    if (idx_bytes < index_bytes(n, B, T) || work_bytes < 256) return kCap;
+    if (idx_bytes < index_bytes(n, B, T, mode) || work_bytes < 256) return kCap;
     if (len < g.body) return kArg;
     uint32_t *ctl = reinterpret_cast<uint32_t *>(d_work);
     SZP_LAUNCH("k_clear", k_clear, dim3(1), dim3(kWG), 0, stream, ctl, nullptr, 0);
     if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;
+    if (mode == 2 && B == 1) {
+        // As in the parse path: the reference's decoding of units of one value.
+        SZP_LAUNCH("k_decode_ra_unit1", k_decode_ra_unit1<V>,
+                   dim3((uint32_t)std::min<uint64_t>((n + kWG - 1) / kWG, 8192)), dim3(kWG), 0,
+                   stream, d_stream, len, g, d_out, ctl);
+        return hipGetLastError() == hipSuccess ? kOk : kHip;
+    }
     uint64_t *ipos;
     uint32_t *icarry;
     index_layout(g, reinterpret_cast<uint8_t *>(const_cast<void *>(d_index)), &ipos, &icarry);
-    launch_k_decode<true>("k_decode_indexed", g, stream, d_stream, len, ipos, icarry, d_out,
-                          nullptr, ctl);
+    launch_k_decode<V, true>("k_decode_indexed", g, stream, d_stream, len, ipos, icarry, d_out,
+                             nullptr, ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;
+}
+
+int launch_decompress_indexed(int dtype, const uint8_t *d_stream, uint64_t len,
+                              const void *d_index, uint64_t idx_bytes, uint64_t n, int B, int T,
+                              void *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream,
+                              int mode)
+{
+    if (dtype == 0)
+        return launch_decompress_indexed_t<float>(mode, d_stream, len, d_index, idx_bytes, n, B,
+                                                  T, reinterpret_cast<float *>(d_out), d_work,
+                                                  work_bytes, stream);
+    if (dtype == 1)
+        return launch_decompress_indexed_t<double>(mode, d_stream, len, d_index, idx_bytes, n, B,
+                                                   T, reinterpret_cast<double *>(d_out), d_work,
+                                                   work_bytes, stream);
+    return kArg;
 }
 
 // ---------------------------------------------------------------------------
@@ -2447,7 +2647,7 @@ bool batch_layout(const BatchField *f, int F, int B, int T, int decode, BatchLay
     L->smap = off;
     off += align_up(4 * (segs ? segs : 1));
     L->parts = off;
-    off += align_up(sizeof(RangePart) * (segs ? segs : 1));
+    off += align_up(sizeof(RangePart<float>) * (segs ? segs : 1));
     L->tiles = tiles;
     L->segs = segs;
     L->total = off;
@@ -2600,9 +2800,9 @@ int batch_launch(const BatchPlan &P, uint32_t ticket, uint32_t grid, hipStream_t
     if (P.segs) {
         SZP_LAUNCH("k_batch_range_partial", k_batch_range_partial, dim3((uint32_t)P.segs), block, 0,
                    stream, jobs, reinterpret_cast<const uint32_t *>(base + P.smap),
-                   reinterpret_cast<RangePart *>(base + P.parts));
+                   reinterpret_cast<RangePart<float> *>(base + P.parts));
         SZP_LAUNCH("k_batch_range_final", k_batch_range_final, dim3(P.F), block, 0, stream, jobs,
-                   reinterpret_cast<const RangePart *>(base + P.parts));
+                   reinterpret_cast<const RangePart<float> *>(base + P.parts));
     }
 #define SZP_BATCH_COMPRESS_CASE(I, S)                                                            \
     case S: {                                                                                    \

@@ -69,9 +69,22 @@ static void check_compress_tile(const Geom &g, uint64_t tile, std::vector<uint8_
                       (unsigned long long)tile, r, d, o, off, (unsigned long long)s.e0, s.m);
                 CHECK(Et + off >= 1 && Et + off + 8 <= g.n, "simple load out of field");
             }
+            if (s.live && !s.head && s.L == 0) {
+                // A fastMode-2 unit of one value: its block is empty.
+                CHECK(g.mode == 2 && s.m == 0, "empty block outside fastMode 2");
+            }
             if (!s.m) continue;
             CHECK(s.e0 + s.m <= g.n, "octet past the field end");
             if (!s.head) CHECK(s.e0 >= 1, "block octet at element 0");
+            if (!s.head && g.mode == 2) {
+                // The element before a block's octet lies in the same unit.
+                uint32_t t2 = rt;
+                uint64_t j2 = rj;
+                advance(g, t2, j2, d);
+                CHECK((j2 & 1) && s.e0 - 1 >= item_elem(g, t2, j2 - 1),
+                      "block octet reads before its unit");
+                CHECK(s.L + 1 <= g.B, "unit longer than B");
+            }
             CHECK(s.e0 >= rE0 && s.e0 + s.m <= rE1, "octet outside its round");
             if (cover) {
                 for (uint32_t i = 0; i < s.m; ++i) {
@@ -137,10 +150,10 @@ static void check_order(const Geom &g, bool with_last)
     if (!in_use) CHECK(tile_of_block(o, 5) == 5, "identity order");
 }
 
-static void check_geometry(uint64_t n, int B, int T, bool full)
+static void check_geometry(uint64_t n, int B, int T, bool full, uint32_t mode = 1)
 {
     Geom g;
-    if (!make_geom(n, B, T, &g)) {
+    if (!make_geom(n, B, T, &g, 4, mode)) {
         CHECK(false, "make_geom rejected n=%llu B=%d T=%d", (unsigned long long)n, B, T);
         return;
     }
@@ -172,7 +185,10 @@ int main()
     const uint64_t ns[] = {1, 2, 7, 8, 9, 15, 16, 17, 40, 100, 1000, 4097, 20001, 65537, 70000, 300001};
     for (uint64_t n : ns)
         for (int B : Bs)
-            for (int T : Ts) check_geometry(n, B, T, true);
+            for (int T : Ts) {
+                check_geometry(n, B, T, true);
+                check_geometry(n, B, T, true, 2);
+            }
     // Full-size shapes, sampled tiles.
     check_geometry(134217728ull, 128, 16, false);
     check_geometry(134217728ull, 32, 16, false);
@@ -180,6 +196,8 @@ int main()
     check_geometry(4294967296ull, 64, 16, false);
     check_geometry(4294967296ull, 64, 1, false);
     check_geometry(4194307ull, 128, 8, false);
+    check_geometry(134217728ull, 128, 16, false, 2);
+    check_geometry(4294967296ull, 64, 16, false, 2);
     printf("geom_check: %llu checks, %d failures\n", (unsigned long long)g_checks, g_fail);
     return g_fail ? 1 : 0;
 }

@@ -74,6 +74,23 @@ def make_input(kind, n, seed):
     raise ValueError(kind)
 
 
+def make_input_f64(kind, n, seed):
+    """float64 inputs: the float32 generators widened, plus a fine integer
+    perturbation (multiples of 2^-40) so that values use the full double
+    mantissa and the FP64 product x * (1 / eb) rounds; "specials" adds
+    doubles beyond the int32 range of q and NaN / Inf."""
+    base = make_input(kind, n, seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 7919)
+    if kind in ("signedzero", "constant", "nanfirst", "posinf"):
+        return base
+    fine = rng.integers(-(1 << 20), (1 << 20) + 1, size=n, dtype=np.int64).astype(np.float64)
+    x = base + fine * 2.0 ** -40
+    if kind == "specials":
+        for val, cnt in ((1e300, 10), (-1e300, 10), (4294967296.5, 10), (2147483647.75, 10)):
+            x[rng.integers(0, n, cnt)] = val
+    return x
+
+
 # (kind, n, seed, mode, abs_eb, rel, B, T): larger inputs whose reference
 # streams are pinned by SHA-256 only.
 HASH_CASES = [

@@ -47,29 +47,74 @@ def oracle():
         O.szpo_decompress.argtypes = [vp, u64, u64, i32, i32, i32, vp]
         O.szpo_compress_api.argtypes = [vp, u64, i32, dbl, dbl, i32, i32, i32, vp, u64,
                                         ctypes.POINTER(u64)]
+        O.szpo_quantize_f64.restype = ctypes.c_int32
+        O.szpo_quantize_f64.argtypes = [dbl, dbl]
+        O.szpo_rel_bound_f64.restype = dbl
+        O.szpo_rel_bound_f64.argtypes = [vp, u64, dbl]
+        O.szpo_decompress_f64.argtypes = [vp, u64, u64, i32, i32, i32, vp]
+        O.szpo_compress_api_f64.argtypes = [vp, u64, i32, dbl, dbl, i32, i32, i32, vp, u64,
+                                            ctypes.POINTER(u64)]
+        O.szpo_compress_mode.argtypes = [i32, i32, vp, u64, i32, dbl, dbl, i32, i32, i32, vp, u64,
+                                         ctypes.POINTER(u64)]
+        O.szpo_decompress_mode.argtypes = [i32, i32, vp, u64, u64, i32, i32, i32, vp]
+        O.szpo_compress_bound_mode.restype = u64
+        O.szpo_compress_bound_mode.argtypes = [i32, i32, u64, i32, i32]
         _O = O
     return _O
 
 
-def compress(x, mode, abs_eb, rel, B, T, workers=0):
-    """Oracle stream (uint8 array) for szp_compress(1, SZ_FLOAT, x, mode, ...) with team size T."""
+def compress(x, mode, abs_eb, rel, B, T, workers=0, dtype=np.float32):
+    """Oracle stream (uint8 array) for szp_compress(1, SZ_FLOAT or SZ_DOUBLE,
+    x, mode, ...) with team size T."""
     O = oracle()
-    x = np.ascontiguousarray(x, dtype=np.float32)
-    cap = O.szpo_compress_bound(x.size, B, T)
+    x = np.ascontiguousarray(x, dtype=dtype)
+    f64 = x.dtype == np.float64
+    cap = O.szpo_compress_bound(x.size, B, T) + (4 if f64 else 0)
     out = np.empty(cap, np.uint8)
     sz = ctypes.c_uint64()
-    rc = O.szpo_compress_api(x.ctypes.data, x.size, mode, abs_eb, rel, B, T, workers,
-                             out.ctypes.data, cap, ctypes.byref(sz))
+    fn = O.szpo_compress_api_f64 if f64 else O.szpo_compress_api
+    rc = fn(x.ctypes.data, x.size, mode, abs_eb, rel, B, T, workers, out.ctypes.data, cap,
+            ctypes.byref(sz))
     if rc:
         raise ValueError("oracle compress rejected the arguments")
     return out[:sz.value].copy()
 
 
-def decompress(stream, n, B, T, workers=0):
+def decompress(stream, n, B, T, workers=0, dtype=np.float32):
     O = oracle()
     s = np.ascontiguousarray(stream, dtype=np.uint8)
-    out = np.empty(n, np.float32)
-    rc = O.szpo_decompress(s.ctypes.data, s.size, n, B, T, workers, out.ctypes.data)
+    out = np.empty(n, dtype)
+    fn = O.szpo_decompress_f64 if out.dtype == np.float64 else O.szpo_decompress
+    rc = fn(s.ctypes.data, s.size, n, B, T, workers, out.ctypes.data)
+    if rc:
+        raise ValueError("oracle decompress found a malformed stream")
+    return out
+
+
+def compress_mode(fast, x, mode, abs_eb, rel, B, T, workers=0):
+    """Oracle stream for szp_compress(fast, SZ_FLOAT or SZ_DOUBLE by x's
+    dtype, x, mode, ...) with team size T; fast = 1 or 2."""
+    O = oracle()
+    x = np.ascontiguousarray(x)
+    assert x.dtype in (np.float32, np.float64)
+    dt = 1 if x.dtype == np.float64 else 0
+    cap = O.szpo_compress_bound_mode(fast, dt, x.size, B, T)
+    out = np.empty(cap, np.uint8)
+    sz = ctypes.c_uint64()
+    rc = O.szpo_compress_mode(fast, dt, x.ctypes.data, x.size, mode, abs_eb, rel, B, T, workers,
+                              out.ctypes.data, cap, ctypes.byref(sz))
+    if rc:
+        raise ValueError("oracle compress rejected the arguments")
+    return out[:sz.value].copy()
+
+
+def decompress_mode(fast, stream, n, B, T, dtype=np.float32, workers=0):
+    O = oracle()
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    out = np.empty(n, dtype)
+    dt = 1 if out.dtype == np.float64 else 0
+    rc = O.szpo_decompress_mode(fast, dt, s.ctypes.data, s.size, n, B, T, workers,
+                                out.ctypes.data)
     if rc:
         raise ValueError("oracle decompress found a malformed stream")
     return out
@@ -96,31 +141,35 @@ _libc = ctypes.CDLL("libc.so.6")
 _libc.free.argtypes = [ctypes.c_void_p]
 
 
-def ref_fits(stream, n, T):
-    """True when the reference's fixed buffers (4N+4 total, ceil(4N/T) per
-    chunk, SZp/src/szp_float.cc:156,186) hold this stream."""
-    hdr = 4 + 8 * T
-    offs = np.frombuffer(stream[4:hdr].tobytes(), np.uint64).astype(np.int64)
+def ref_fits(stream, n, T, es=4):
+    """True when the reference's fixed buffers (es*N+es total, ceil(es*N/T)
+    per chunk, SZp/src/szp_float.cc:156,186, SZp/src/szp_double.cc:157,186)
+    hold this stream; es = bytes per value."""
+    hdr = es + 8 * T
+    offs = np.frombuffer(stream[es:hdr].tobytes(), np.uint64).astype(np.int64)
     sizes = np.diff(np.append(offs, stream.size - hdr))
-    return stream.size <= 4 * n + 4 and sizes.max() <= (4 * n + T - 1) // T
+    return stream.size <= es * n + es and sizes.max() <= (es * n + T - 1) // T
 
 
-def ref_compress(x, mode, abs_eb, rel, B, T):
+def ref_compress(x, mode, abs_eb, rel, B, T, dtype=np.float32, fast=1):
     R = reference()
     R.szp_ref_set_threads(T)
-    x = np.ascontiguousarray(x, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=dtype)
     sz = ctypes.c_size_t()
-    p = R.szp_compress(1, 0, x.ctypes.data, ctypes.byref(sz), mode, abs_eb, rel, x.size, B)
+    p = R.szp_compress(fast, 1 if x.dtype == np.float64 else 0, x.ctypes.data, ctypes.byref(sz),
+                       mode, abs_eb, rel, x.size, B)
     out = np.ctypeslib.as_array((ctypes.c_uint8 * sz.value).from_address(p)).copy()
     _libc.free(p)
     return out
 
 
-def ref_decompress(stream, n, B, T):
+def ref_decompress(stream, n, B, T, dtype=np.float32, fast=1):
     R = reference()
     R.szp_ref_set_threads(T)
     s = np.ascontiguousarray(stream, dtype=np.uint8)
-    p = R.szp_decompress(1, 0, s.ctypes.data, s.size, n, B)
-    out = np.ctypeslib.as_array((ctypes.c_float * n).from_address(p)).copy()
+    f64 = np.dtype(dtype) == np.float64
+    p = R.szp_decompress(fast, 1 if f64 else 0, s.ctypes.data, s.size, n, B)
+    ct = ctypes.c_double if f64 else ctypes.c_float
+    out = np.ctypeslib.as_array((ct * n).from_address(p)).copy()
     _libc.free(p)
     return out

@@ -93,10 +93,50 @@ def test_unsupported_modes_return_null_without_a_device(lib):
     import szp_amd
     x = np.zeros(16, np.float32)
     with pytest.raises(szp_amd.SzpError):
-        szp_amd.szp_compress(2, 0, x, 0, 0.1, 0.0, 16, 8)  # random-access mode
-    with pytest.raises(szp_amd.SzpError):
-        szp_amd.szp_compress(1, 1, x, 0, 0.1, 0.0, 16, 8)  # double
+        szp_amd.szp_compress(3, 0, x, 0, 0.1, 0.0, 16, 8)  # no such fastMode
     assert b"supported" in szp_amd.lib.szp_amd_last_error()
+    size = ctypes.c_size_t()
+    L = szp_amd.lib
+    assert not L.szp_compress(1, 2, x.ctypes.data, ctypes.byref(size), 0, 0.1, 0.0, 16, 8)
+    assert b"supported" in L.szp_amd_last_error()
+    assert not L.szp_decompress(1, 7, x.ctypes.data, 64, 16, 8)
+    with pytest.raises(szp_amd.SzpError):
+        szp_amd.szp_compress(1, 2, x, 0, 0.1, 0.0, 16, 8)  # the mirror rejects it too
+
+
+def test_ra_bound_and_stream_length():
+    import json
+    import szp_amd
+    O = oracle_lib.oracle()
+    for n in (0, 1, 100, 4097, 1 << 20):
+        for B in (1, 2, 32, 128):
+            for T in (1, 16):
+                for dt in (0, 1):
+                    assert (szp_amd.compress_bound(n, B, T, dt, 2)
+                            == O.szpo_compress_bound_mode(2, dt, n, B, T))
+    d = os.path.join(ROOT, "tests", "golden")
+    cases = json.load(open(os.path.join(d, "cases_ra.json")))
+    arrays = np.load(os.path.join(d, "cases_ra.npz"))
+    for c in cases:
+        s = arrays["stream_%d" % c["id"]]
+        assert szp_amd.stream_length(s, c["n"], c["B"], c["T"], c["dtype"], 2) == s.size
+
+
+def test_f64_bound_and_stream_length():
+    import json
+    import szp_amd
+    O = oracle_lib.oracle()
+    for n in (0, 1, 100, 4097, 1 << 20):
+        for B in (1, 32, 128):
+            for T in (1, 16):
+                assert (szp_amd.compress_bound(n, B, T, szp_amd.SZ_DOUBLE)
+                        == O.szpo_compress_bound(n, B, T) + 4)
+    d = os.path.join(ROOT, "tests", "golden")
+    cases = json.load(open(os.path.join(d, "cases_f64.json")))
+    arrays = np.load(os.path.join(d, "cases_f64.npz"))
+    for c in cases:
+        s = arrays["stream_%d" % c["id"]]
+        assert szp_amd.stream_length(s, c["n"], c["B"], c["T"], szp_amd.SZ_DOUBLE) == s.size
 
 
 def test_sizes_are_consistent():
