@@ -18,7 +18,7 @@ constexpr int kWG = 256;                 // threads per workgroup (4 wave64)
 constexpr int kMaxBlock = 2048;          // largest SZp block size handled on the GPU
 constexpr int kOutBufBytes = 8704 + 64;  // LDS bytes for one round's encoded blocks
 constexpr int kMaxRounds = 4;            // rounds of K items per tile
-constexpr int kOutCap = 28672;           // LDS bytes buffering a compress tile before placement
+constexpr int kOutCap = 20480;           // LDS bytes buffering a compress tile before placement
 constexpr int kDecTileBytes = 20480;     // LDS bytes holding a decode tile's stream bytes
 
 // Geometry of the item sequence of one field.  The stream body is the

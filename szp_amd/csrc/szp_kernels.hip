@@ -365,12 +365,17 @@ __device__ __forceinline__ void vec_minmax(const uint8_t *p, double &mn, double 
 // is ragged).
 constexpr int kRangeUnroll = 4;
 
+// Segments are taken from the field's end backwards (workgroup b reads
+// segment gridDim.x - 1 - b): the field's first bytes, which the compress
+// kernel reads first, are the ones read last here and are still in the
+// memory-side cache (256 MB on MI355X) when it starts.
 template <typename V, bool VEC>
 __global__ __launch_bounds__(kWG) void k_range_partial(const V *__restrict__ x, uint64_t n,
                                                        uint64_t seg, RangePart<V> *part)
 {
     constexpr uint32_t kPer = 16 / sizeof(V);  // values per 16-byte load
-    const uint64_t s = (uint64_t)blockIdx.x * seg;
+    const uint32_t sg = gridDim.x - 1 - blockIdx.x;
+    const uint64_t s = (uint64_t)sg * seg;
     const uint32_t len = (uint32_t)(min(s + seg, n) - s);
     const V *xs = x + s;
     V mn = (V)__builtin_nan(""), mx = (V)__builtin_nan("");
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(kWG) void k_range_partial(const V *__restrict__ x, 
         }
     }
     range_reduce_wg(mn, mx);
-    if (threadIdx.x == 0) part[blockIdx.x] = RangePart<V>{mn, mx};
+    if (threadIdx.x == 0) part[sg] = RangePart<V>{mn, mx};
 }
 
 // eb = (max - min) * rel in the value type (float: fl32(fl32(max - min) *
@@ -1089,8 +1094,10 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     }
 }
 
+// Five workgroups per CU: at most 96 VGPRs (amdgpu_waves_per_eu) and the
+// LDS of CompressLds (kOutCap) sized to match.
 template <typename V, bool INDEX, int SPB>
-__global__ __launch_bounds__(kWG) void k_compress(const V *__restrict__ x,
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_compress(const V *__restrict__ x,
                                                   const V *__restrict__ d_eb, V eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
                                                   uint64_t *__restrict__ status,
