@@ -1094,10 +1094,11 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     }
 }
 
-// Five workgroups per CU: at most 96 VGPRs (amdgpu_waves_per_eu) and the
-// LDS of CompressLds (kOutCap) sized to match.
+// Float fields: five workgroups per CU (at most 96 VGPRs; the LDS of
+// CompressLds, kOutCap, is sized to match).  Double fields keep more
+// registers (their octets are twice as wide) and run fewer workgroups.
 template <typename V, bool INDEX, int SPB>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_compress(const V *__restrict__ x,
+__global__ __launch_bounds__(kWG, sizeof(V) == 4 ? 5 : 1) void k_compress(const V *__restrict__ x,
                                                   const V *__restrict__ d_eb, V eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
                                                   uint64_t *__restrict__ status,
