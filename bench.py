@@ -419,6 +419,71 @@ def serving_bench(torch, szp, rate=2000.0, count=400, pool_size=48, B=128, T=16,
     return res
 
 
+def other_paths_bench(torch, szp, x, rel=1e-4, B=128, T=16, steps=10, cpu_check=True):
+    """The 512^3 field through the float64 path (the field widened to double)
+    and through fastMode 2 (random access, float): compress with the side
+    index, decompress with it and as a foreign stream; stream bytes and
+    decompressed values against the reference on the same field and T."""
+    import numpy as np
+    n = x.numel()
+    out = {}
+    cases = (("double", x.double().contiguous(), 1), ("random_access", x, 2))
+    for name, xv, mode in cases:
+        es = xv.element_size()
+        codec = szp.DeviceCodec(block_size=B, threads=T, fast_mode=mode)
+        st, size_t, idx = codec.compress(xv, szp.REL, 0.0, rel, index=True)
+        codec.check()
+        C = int(size_t.item())
+        dec = torch.empty_like(xv)
+        for _ in range(2):
+            codec.compress(xv, szp.REL, 0.0, rel, out=st, index=idx)
+            codec.decompress(st, C, n, out=dec, index=idx)
+            codec.decompress(st, C, n, out=dec)
+        codec.check()
+        comp_ms = _timed(torch, lambda: codec.compress(xv, szp.REL, 0.0, rel, out=st, index=idx), steps)
+        dec_ms = _timed(torch, lambda: codec.decompress(st, C, n, out=dec, index=idx), steps)
+        parse_ms = _timed(torch, lambda: codec.decompress(st, C, n, out=dec), steps)
+        codec.check()
+        r = {"workload": "512^3 k^-3 field (%s), %s, REL %g, B=%d, T=%d" % (
+                 "float64" if es == 8 else "float32", "fastMode %d" % mode, rel, B, T),
+             "compressed_bytes": C, "ratio": round(es * n / C, 4),
+             "compress_GBps": round(es * n / (comp_ms * 1e-3) / 1e9, 1),
+             "compress_hbm_frac": round((2 * es * n + C) / (comp_ms * 1e-3) / 8e12, 4),
+             "decompress_GBps": round(es * n / (dec_ms * 1e-3) / 1e9, 1),
+             "decompress_hbm_frac": round((C + es * n) / (dec_ms * 1e-3) / 8e12, 4),
+             "decompress_foreign_stream_GBps": round(es * n / (parse_ms * 1e-3) / 1e9, 1)}
+        if cpu_check:
+            kind, _, _ = load_reference(T)
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+            xh = xv.cpu().numpy()
+            t0 = time.perf_counter()
+            if kind == "reference":
+                ref = oracle_lib.ref_compress(xh, 1, 0.0, rel, B, T, dtype=xh.dtype, fast=mode)
+                t1 = time.perf_counter()
+                ref_dec = oracle_lib.ref_decompress(ref, n, B, T, dtype=xh.dtype, fast=mode)
+            else:
+                ref = oracle_lib.compress_mode(mode, xh, 1, 0.0, rel, B, T, workers=T)
+                t1 = time.perf_counter()
+                ref_dec = oracle_lib.decompress_mode(mode, ref, n, B, T, dtype=xh.dtype, workers=T)
+            t2 = time.perf_counter()
+            got = st[:C].cpu().numpy()
+            dh = dec.cpu().numpy()
+            ut = np.uint64 if es == 8 else np.uint32
+            r["stream_byte_identical"] = bool(ref.size == C and np.array_equal(ref, got))
+            r["decompress_bit_identical"] = bool(np.array_equal(ref_dec.view(ut), dh.view(ut)))
+            r["against"] = kind
+            r["cpu_compress_GBps_%d_threads" % T] = round(es * n / (t1 - t0) / 1e9, 3)
+            r["cpu_decompress_GBps_%d_threads" % T] = round(es * n / (t2 - t1) / 1e9, 3)
+            del xh, ref, ref_dec, got, dh
+        out[name] = r
+        del codec, st, idx, dec
+        log("%s leg: compress %.1f GB/s, decompress %.1f GB/s" % (name, r["compress_GBps"],
+                                                                  r["decompress_GBps"]))
+    torch.cuda.empty_cache()
+    return out
+
+
 def _blocks(n, B, T):
     """SZp blocks of a field: each chunk of n_t elements has ceil((n_t - 1) / B)."""
     tbs = n // T
@@ -546,6 +611,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the serving leg (JSON line)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving arrivals per second")
     ap.add_argument("--no-large", action="store_true", help="skip the 2^32-element field leg")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the float64 and fastMode 2 (random access) legs")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 2^32-element field (16 GiB) leg (JSON line)")
     ap.add_argument("--batch-only", action="store_true", help="only the batched leg (JSON line)")
@@ -656,6 +723,11 @@ def main():
             variants.append(v)
             del vc, vr
 
+    other = None
+    if not args.no_other:
+        log("float64 and random-access legs")
+        other = other_paths_bench(torch, szp, x, rel=rel, B=B, T=T, cpu_check=not args.no_cpu)
+
     batched = None
     if not args.no_batch:
         log("batched leg: 256 fields")
@@ -706,6 +778,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
+        "other_paths": other,
         "batched": batched,
         "serving": served,
         "large_field": large,

@@ -1,4 +1,5 @@
-"""Builds libszp_amd.so (HIP kernels + C ABI) in-tree for gfx950 with hipcc.
+"""Builds libszp_amd.so (HIP kernels + C ABI) in-tree for gfx950 with hipcc,
+and the drop-in command-line drivers (szp_amd/bin) linked against it.
 
     python szp_amd/build.py      # or __graft_entry__.build(); runs without importing the package
 """
@@ -26,6 +27,18 @@ def hipcc_command(out=LIB):
     ]
 
 
+CLI = ["testfloat_compress_fastmode1", "testfloat_decompress_fastmode1"]
+BIN = os.path.join(PKG, "bin")
+
+
+def cli_command(name):
+    """g++ line of one drop-in command-line driver (szp_amd/cli), linked
+    against the in-tree library (found through an $ORIGIN rpath)."""
+    return ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+            os.path.join(PKG, "cli", name + ".cc"), "-L", PKG, "-lszp_amd",
+            "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BIN, name)]
+
+
 def build(verbose=False):
     tmp = LIB + ".tmp"
     cmd = hipcc_command(tmp)
@@ -36,6 +49,15 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libszp_amd.so")
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    for name in CLI:
+        c = cli_command(name)
+        if verbose:
+            print(" ".join(c), flush=True)
+        r = subprocess.run(c, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("g++ failed building " + name)
     return LIB
 
 

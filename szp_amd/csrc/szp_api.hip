@@ -17,6 +17,7 @@
 
 #include "szp.h"
 #include "szp_amd.h"
+#include "szp_rw.h"
 #include "szp_internal.h"
 
 using namespace szp_amd;
@@ -710,6 +711,119 @@ void szp_double_decompress_openmp_threadblock_randomaccess_arg(double *newData, 
     }
     lower_decompress("szp_double_decompress_openmp_threadblock_randomaccess_arg", 2, SZ_DOUBLE,
                      absErrBound, nbEle, blockSize, cmpBytes, newData);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Raw-file helpers (include/szp_rw.h)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Reads a whole file; nullptr (status SZ_FERR) when it cannot be opened or
+// read.
+unsigned char *read_file(const char *path, size_t *length, int *status)
+{
+    *status = SZ_FERR;
+    FILE *f = fopen(path, "rb");
+    if (!f) return nullptr;
+    if (fseek(f, 0, SEEK_END) != 0) {
+        fclose(f);
+        return nullptr;
+    }
+    const long end = ftell(f);
+    if (end < 0 || fseek(f, 0, SEEK_SET) != 0) {
+        fclose(f);
+        return nullptr;
+    }
+    const size_t n = (size_t)end;
+    unsigned char *buf = (unsigned char *)malloc(n ? n : 1);
+    if (!buf || fread(buf, 1, n, f) != n) {
+        free(buf);
+        fclose(f);
+        return nullptr;
+    }
+    fclose(f);
+    *length = n;
+    *status = SZ_SCES;
+    return buf;
+}
+
+// The values of a data file as host-order values of `width` bytes, swapping
+// each value's bytes when the data's byte order differs from the host's
+// (SZp/src/szp_rw.cc:424-459).
+void *read_values(const char *path, size_t width, size_t *nbEle, int *status)
+{
+    size_t len = 0;
+    unsigned char *buf = read_file(path, &len, status);
+    if (!buf) return nullptr;
+    *nbEle = len / width;
+    if (szp_dataEndianType != szp_sysEndianType) {
+        for (size_t i = 0; i < *nbEle; ++i)
+            std::reverse(buf + i * width, buf + (i + 1) * width);
+    }
+    return buf;
+}
+
+void write_file(const void *data, size_t length, const char *path, int *status)
+{
+    *status = SZ_FERR;
+    FILE *f = fopen(path, "wb");
+    if (!f) return;
+    const bool ok = length == 0 || fwrite(data, 1, length, f) == length;
+    if (fclose(f) == 0 && ok) *status = SZ_SCES;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t szp_checkFileSize(char *srcFilePath, int *status)
+{
+    *status = SZ_FERR;
+    FILE *f = fopen(srcFilePath, "rb");
+    if (!f) return 0;
+    size_t n = 0;
+    if (fseek(f, 0, SEEK_END) == 0) {
+        const long end = ftell(f);
+        if (end >= 0) {
+            n = (size_t)end;
+            *status = SZ_SCES;
+        }
+    }
+    fclose(f);
+    return n;
+}
+
+unsigned char *szp_readByteData(char *srcFilePath, size_t *byteLength, int *status)
+{
+    return read_file(srcFilePath, byteLength, status);
+}
+
+float *szp_readFloatData(char *srcFilePath, size_t *nbEle, int *status)
+{
+    return (float *)read_values(srcFilePath, sizeof(float), nbEle, status);
+}
+
+double *szp_readDoubleData(char *srcFilePath, size_t *nbEle, int *status)
+{
+    return (double *)read_values(srcFilePath, sizeof(double), nbEle, status);
+}
+
+void szp_writeByteData(unsigned char *bytes, size_t byteLength, char *tgtFilePath, int *status)
+{
+    write_file(bytes, byteLength, tgtFilePath, status);
+}
+
+void szp_writeFloatData_inBytes(float *data, size_t nbEle, char *tgtFilePath, int *status)
+{
+    write_file(data, nbEle * sizeof(float), tgtFilePath, status);
+}
+
+void szp_writeDoubleData_inBytes(double *data, size_t nbEle, char *tgtFilePath, int *status)
+{
+    write_file(data, nbEle * sizeof(double), tgtFilePath, status);
 }
 
 }  // extern "C"

@@ -17,7 +17,7 @@ LIB = os.path.join(ROOT, "szp_amd", "libszp_amd.so")
 
 def declared_functions():
     names = set()
-    for h in ("szp.h", "szp_amd.h"):
+    for h in ("szp.h", "szp_amd.h", "szp_rw.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
@@ -146,3 +146,51 @@ def test_sizes_are_consistent():
         assert L.szp_amd_compress_workspace_size(n, B, T) > 0
         assert L.szp_amd_decompress_workspace_size(szp_amd.compress_bound(n, B, T), n, B, T) > 0
         assert L.szp_amd_index_size(n, B, T) >= 8 * ((n + B - 1) // B)
+
+
+def test_raw_file_helpers(tmp_path):
+    """szp_rw.h: byte files round-trip; float/double files are read in the
+    byte order szp_dataEndianType names (swapped when it differs from the
+    host's, SZp/src/szp_rw.cc:424-459)."""
+    L = ctypes.CDLL(LIB)
+    L.szp_readByteData.restype = ctypes.c_void_p
+    L.szp_readFloatData.restype = ctypes.c_void_p
+    L.szp_readDoubleData.restype = ctypes.c_void_p
+    L.szp_checkFileSize.restype = ctypes.c_size_t
+    libc = ctypes.CDLL("libc.so.6")
+    libc.free.argtypes = [ctypes.c_void_p]
+    st = ctypes.c_int(7)
+    n = ctypes.c_size_t(0)
+    data = np.arange(1000, dtype=np.uint8)
+    f = str(tmp_path / "b.bin").encode()
+    L.szp_writeByteData(data.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(data.size), f,
+                        ctypes.byref(st))
+    assert st.value == 0
+    assert L.szp_checkFileSize(f, ctypes.byref(st)) == 1000 and st.value == 0
+    p = L.szp_readByteData(f, ctypes.byref(n), ctypes.byref(st))
+    assert st.value == 0 and n.value == 1000
+    assert bytes((ctypes.c_uint8 * 1000).from_address(p)) == data.tobytes()
+    libc.free(p)
+    x = np.linspace(-1, 1, 257).astype(np.float32)
+    g = str(tmp_path / "x.f32").encode()
+    L.szp_writeFloatData_inBytes(x.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(x.size), g,
+                                 ctypes.byref(st))
+    assert st.value == 0 and open(g, "rb").read() == x.tobytes()
+    p = L.szp_readFloatData(g, ctypes.byref(n), ctypes.byref(st))
+    assert n.value == 257 and np.array_equal(np.ctypeslib.as_array((ctypes.c_float * 257).from_address(p)), x)
+    libc.free(p)
+    y = x.astype(np.float64)
+    h = str(tmp_path / "y.f64").encode()
+    L.szp_writeDoubleData_inBytes(y.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(y.size), h,
+                                  ctypes.byref(st))
+    endian = ctypes.c_int.in_dll(L, "szp_dataEndianType")
+    endian.value = 1  # big-endian data on a little-endian host: values are swapped
+    try:
+        p = L.szp_readDoubleData(h, ctypes.byref(n), ctypes.byref(st))
+        got = np.ctypeslib.as_array((ctypes.c_double * 257).from_address(p)).copy()
+        libc.free(p)
+    finally:
+        endian.value = 0
+    assert np.array_equal(got.view(np.uint64), y.byteswap().view(np.uint64))
+    assert L.szp_readByteData(str(tmp_path / "missing").encode(), ctypes.byref(n),
+                              ctypes.byref(st)) is None and st.value == -2
