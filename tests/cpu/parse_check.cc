@@ -10,8 +10,9 @@
 // For each stream it compares the recovered block positions with a plain
 // serial walk of the chunks, and the kernel's stream-error verdict with the
 // serial walk's.  Streams are read from a case list written by the test:
-//   parse_check <list>   lines: "<file> <n> <B> <T>"
+//   parse_check <list>   lines: "<file> <n> <B> <T> <fastMode>"
 // (the stream length is the file size, which may exceed the real stream).
+// fastMode 2 streams are parsed unit by unit (a 4-byte head and a block).
 #include <stdio.h>
 #include <stdlib.h>
 #include <stdint.h>
@@ -31,10 +32,14 @@ static uint64_t rd64(const uint8_t *p)
     return v;
 }
 
-// Sub-tile bytes, as the launch code sizes them.
-static uint32_t subtile_bytes(uint32_t B)
+// Longest chain step and sub-tile bytes, as the launch code sizes them.
+static uint32_t max_step(uint32_t B, uint32_t mode)
 {
-    const uint32_t W = block_bytes(32, B);
+    return mode == 2 ? 4 + block_bytes(32, B - 1) : block_bytes(32, B);
+}
+
+static uint32_t subtile_bytes(uint32_t W)
+{
     const uint32_t U = W > 4096 ? W : 4096;
     return (U + 255) & ~255u;
 }
@@ -72,6 +77,34 @@ static Result serial_parse(const uint8_t *st, uint64_t len, const Geom &g)
             return r;
         }
         const uint64_t nb = (t == g.T - 1) ? g.nblast : g.nb0;
+        if (g.mode == 2) {
+            // Units: [q i32][width byte][...]; a one-value unit's width byte
+            // is not read.
+            uint64_t p = cs;
+            if (cs + 5 > ce) {
+                r.bad = true;
+                return r;
+            }
+            for (uint64_t j = 0; j < nb; ++j) {
+                if (p + 4 >= len || p >= ce) {
+                    r.bad = true;
+                    return r;
+                }
+                const uint64_t L = ((j + 1 < nb) ? g.B : nt - (nb - 1) * (uint64_t)g.B) - 1;
+                const uint32_t b = st[p + 4];
+                if ((L || j + 1 < nb) && b > 32) {
+                    r.bad = true;
+                    return r;
+                }
+                r.pos[(uint64_t)t * g.nb0 + j] = p;
+                p += 4 + block_bytes(L ? b : 0, (uint32_t)L);
+            }
+            if (nb && (t + 1 < g.T ? p != ce : p > len)) {
+                r.bad = true;
+                return r;
+            }
+            continue;
+        }
         uint64_t p = cs + 4;
         for (uint64_t j = 0; j < nb; ++j) {
             if (p >= ce) {
@@ -104,9 +137,9 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
     const uint64_t nblocks = (uint64_t)(g.T - 1) * g.nb0 + g.nblast;
     r.pos.assign(nblocks, ~0ull);
     r.written.assign(nblocks, 0);
-    const uint32_t B = g.B;
-    const uint32_t U = subtile_bytes(B), S = 4 * U, NW = S / 32;
-    const uint32_t W = block_bytes(32, B);
+    const uint32_t B = g.B, mode = g.mode;
+    const uint32_t W = max_step(B, mode);
+    const uint32_t U = subtile_bytes(W), S = 4 * U, NW = S / 32;
 
     // k_dsetup
     std::vector<uint64_t> cstart(g.T), aend(g.T), ttab(g.T + 1);
@@ -124,13 +157,14 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
             continue;
         }
         const uint64_t ce = (t + 1 < g.T) ? g.body + rd64(st + 4 + 8ull * (t + 1)) : len;
-        if (off > len || cs + 4 > ce || ce > len) {
+        const uint64_t a0 = mode == 2 ? cs : cs + 4;
+        if (off > len || a0 + (mode == 2 ? 5 : 0) > ce || a0 > ce || ce > len) {
             bad = true;
             break;
         }
         aend[t] = ce;
         const uint64_t nb = (t == g.T - 1) ? g.nblast : g.nb0;
-        const uint64_t alen = ce - (cs + 4);
+        const uint64_t alen = ce - a0;
         if (nb && !alen) bad = true;
         if (nb) tiles += (alen + S - 1) / S;
     }
@@ -147,7 +181,7 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
         while (t + 1 < g.T && ttab[t + 1] <= gt) ++t;
         const uint64_t first = ttab[t];
         const uint64_t i = gt - first;
-        const uint64_t as = cstart[t] + 4, ae = aend[t];
+        const uint64_t as = cstart[t] + (mode == 2 ? 0 : 4), ae = aend[t];
         const uint64_t s = as + i * S;
         const uint64_t te = std::min<uint64_t>(s + S, ae);
         const uint32_t span = (uint32_t)(te - s);
@@ -159,7 +193,8 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
         std::unique_ptr<uint32_t[]> spec(new uint32_t[NW]), pre(new uint32_t[NW]);
         std::unique_ptr<uint16_t[]> rspec(new uint16_t[NW]), rpre(new uint16_t[NW]);
         memset(buf.get(), 0xA5, S + 64);  // stale LDS contents
-        for (uint64_t a = s0; a < te; a += 16) {
+        const uint64_t tl = std::min<uint64_t>(te + 16, len);
+        for (uint64_t a = s0; a < tl; a += 16) {
             if (a + 16 <= len) {
                 memcpy(buf.get() + (a - s0), st + a, 16);
             } else {
@@ -184,8 +219,8 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
             const uint32_t cmax = std::min(u + W, v);
             for (uint32_t c = u; c < cmax && !ok; ++c) {
                 uint32_t e = 0;
-                if (parse_chain_leaves(tb, c, v, B, e)) {
-                    cnt = parse_mark_chain(tb, spec.get(), c, v, B);
+                if (parse_chain_leaves(tb, c, v, B, e, mode)) {
+                    cnt = parse_mark_chain(tb, spec.get(), c, v, B, mode);
                     ex = e;
                     ok = true;
                 }
@@ -203,7 +238,7 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
             } else {
                 const uint32_t cmax = std::min(u + W, v);
                 for (uint32_t c = u; c < cmax; ++c) {
-                    const uint32_t leave = parse_cand_walk(tb, spec.get(), c, v, B);
+                    const uint32_t leave = parse_cand_walk(tb, spec.get(), c, v, B, mode);
                     if (leave && s + leave != s_exit[lastw]) s_nonsync = 1;
                 }
             }
@@ -234,7 +269,7 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
                 s_broken[w] = broken;
                 if (uw >= span || broken || e >= s + vw) continue;
                 uint32_t p = (uint32_t)(e - s), npre = 0;
-                const int res = parse_true_walk(tb, spec.get(), pre.get(), p, vw, B, npre);
+                const int res = parse_true_walk(tb, spec.get(), pre.get(), p, vw, B, npre, mode);
                 s_npre[w] = npre;
                 if (res == 1) {
                     s_merged[w] = 1;
@@ -310,9 +345,18 @@ static Result emulate(const uint8_t *st, uint64_t len, const Geom &g)
         if (gt + 1 == ttab[t + 1] && prefix + tile_cnt < nbt) r.bad = true;
         if (s_lastpos != ~0ull) {
             const uint64_t pl = s_lastpos;
-            const uint32_t b = st[pl];
-            const uint64_t Ll = chunk_len(g, t) - 1 - (nbt - 1) * (uint64_t)g.B;
-            const uint64_t end = pl + block_bytes(b, (uint32_t)Ll);
+            uint64_t end;
+            uint32_t b;
+            if (mode == 2) {
+                const uint64_t Ll = chunk_len(g, t) - (nbt - 1) * (uint64_t)g.B - 1;
+                b = pl + 4 < len ? st[pl + 4] : 255u;
+                end = pl + 4 + block_bytes(b, (uint32_t)Ll);
+                if (!Ll && pl + 4 < len) b = 0;
+            } else {
+                const uint64_t Ll = chunk_len(g, t) - 1 - (nbt - 1) * (uint64_t)g.B;
+                b = st[pl];
+                end = pl + block_bytes(b, (uint32_t)Ll);
+            }
             if (b > 32 || (t + 1 < g.T ? end != ae : end > len)) r.bad = true;
         }
     }
@@ -329,8 +373,8 @@ int main(int argc, char **argv)
     if (!f) return 2;
     char path[4096];
     unsigned long long n;
-    int B, T, ncase = 0, fail = 0, nbad = 0;
-    while (fscanf(f, "%4095s %llu %d %d", path, &n, &B, &T) == 4) {
+    int B, T, mode, ncase = 0, fail = 0, nbad = 0;
+    while (fscanf(f, "%4095s %llu %d %d %d", path, &n, &B, &T, &mode) == 5) {
         FILE *sf = fopen(path, "rb");
         if (!sf) return 2;
         fseek(sf, 0, SEEK_END);
@@ -340,7 +384,7 @@ int main(int argc, char **argv)
         if (fread(st.data(), 1, (size_t)len, sf) != (size_t)len) return 2;
         fclose(sf);
         Geom g;
-        if (!make_geom(n, B, T, &g)) {
+        if (!make_geom(n, B, T, &g, 4, (uint32_t)mode)) {
             fprintf(stderr, "bad geometry %s\n", path);
             return 2;
         }

@@ -83,10 +83,12 @@ def _run(parse_check, entries):
     exe, d = parse_check
     lst = d / ("cases_%d.txt" % len(os.listdir(d)))
     lines = []
-    for k, (st, n, B, T) in enumerate(entries):
+    for k, e in enumerate(entries):
+        st, n, B, T = e[:4]
+        mode = e[4] if len(e) > 4 else 1
         p = d / ("s_%s_%d.bin" % (lst.stem, k))
         np.ascontiguousarray(st, dtype=np.uint8).tofile(p)
-        lines.append("%s %d %d %d" % (p, n, B, T))
+        lines.append("%s %d %d %d %d" % (p, n, B, T, mode))
     lst.write_text("\n".join(lines) + "\n")
     r = subprocess.run([exe, str(lst)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr[-4000:]
@@ -161,3 +163,73 @@ def test_parse_damaged_streams(parse_check):
     out = _run(parse_check, entries)
     ninv = int(out.split("(")[1].split()[0])
     assert ninv > 40, out
+
+
+# fastMode 2 (units): B >= 2 (B = 1 streams decode without a parse).
+VALID_RA = [
+    ("waves", 1 << 20, 128, 4, 1, 0.0, 1e-4),
+    ("waves", 1 << 20, 32, 3, 1, 0.0, 1e-2),
+    ("waves", (1 << 19) + 7, 2, 2, 1, 0.0, 1e-3),
+    ("walk", 300001, 2048, 2, 1, 0.0, 1e-4),
+    ("walk", 200001, 9, 5, 0, 0.01, 0.0),
+    ("uniform", 400000, 128, 3, 0, 0.002, 0.0),
+    ("zeroblocks", 800003, 64, 3, 1, 0.0, 1e-3),
+    ("constant", 2000000, 8, 2, 1, 0.0, 1e-3),
+    ("specials", 300000, 32, 4, 0, 0.5, 0.0),
+    ("sparsewide", 400000, 64, 2, 0, 1.0, 0.0),
+    ("walk", 41, 8, 16, 0, 0.01, 0.0),
+    ("walk", 17, 8, 2, 0, 0.01, 0.0),
+]
+
+
+def _ra_stream(case, seed):
+    kind, n, B, T, mode, ab, rel = case
+    return oracle_lib.compress_mode(2, make_input(kind, n, seed), mode, ab, rel, B, T)
+
+
+def _unit_width_positions(st, n, B, T):
+    """Positions of the width byte of every unit with more than one value."""
+    offs = [int.from_bytes(st[4 + 8 * t:12 + 8 * t].tobytes(), "little") for t in range(T)]
+    body = 4 + 8 * T
+    per = n // T
+    O = oracle_lib.oracle()
+    out = []
+    for t in range(T):
+        nt = per if t < T - 1 else n - per * (T - 1)
+        p = body + offs[t]
+        for i in range(0, nt, B):
+            L = min(B, nt - i) - 1
+            if L:
+                out.append(p + 4)
+                p += 4 + O.szpo_block_bytes(int(st[p + 4]), L)
+            else:
+                p += 5
+    return out
+
+
+def test_parse_random_access_streams(parse_check):
+    """fastMode 2: valid streams, garbage tails, truncation, flipped width
+    bytes and bad chunk offsets, against the serial walk of the units."""
+    rng = np.random.default_rng(5)
+    entries = []
+    for k, c in enumerate(VALID_RA):
+        kind, n, B, T = c[0], c[1], c[2], c[3]
+        st = _ra_stream(c, 200 + k)
+        entries.append((st, n, B, T, 2))
+        entries.append((np.concatenate([st, rng.integers(0, 256, 5000, dtype=np.uint8)]), n, B, T, 2))
+        entries.append((st[:-1].copy(), n, B, T, 2))
+        widths = _unit_width_positions(st, n, B, T)
+        for trial in range(4):
+            s2 = st.copy()
+            if widths:
+                p = widths[int(rng.integers(0, len(widths)))]
+                s2[p] = (33 + int(rng.integers(0, 223))) if trial % 2 == 0 else int(rng.integers(0, 33))
+            entries.append((s2, n, B, T, 2))
+        if T > 1:
+            s3 = st.copy()
+            off = int.from_bytes(st[12:20].tobytes(), "little")
+            s3[12:20] = np.frombuffer(np.uint64(off + 3).tobytes(), np.uint8)
+            entries.append((s3, n, B, T, 2))
+    out = _run(parse_check, entries)
+    ninv = int(out.split("(")[1].split()[0])
+    assert ninv > 10, out
