@@ -174,6 +174,17 @@ SZP_HD bool simple_tile(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nti)
     return (g.B & 7) == 0 && cj >= 1 && nti >= 1 && cj + nti - 1 <= nfull;
 }
 
+// A fastMode-2 tile of whole units of B = 8 * spb values, all of chunk ct:
+// it starts at a unit's head (cj even) and holds nti / 2 full units.  Its
+// elements are contiguous from item_elem(g, ct, cj), unit d of fast round r
+// (K units per round) at r * 2048 + d * B.
+SZP_HD bool unit_tile(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nti)
+{
+    if (g.mode != 2 || (g.B & 7) || (cj & 1) || (nti & 1) || !nti) return false;
+    const uint64_t nfull = chunk_len(g, ct) / g.B;  // the chunk's full units
+    return cj / 2 + nti / 2 <= nfull;
+}
+
 // Element offset, from a simple tile's first element, of the octet of slot
 // (d, o) in round r.
 SZP_HD uint32_t simple_octet_off(const Geom &g, uint32_t r, uint32_t d, uint32_t o)

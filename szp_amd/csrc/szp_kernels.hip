@@ -622,7 +622,33 @@ __device__ __forceinline__ uint32_t octet8_deltas(V p, const V v[8], V inv, uint
     return omax;
 }
 
-// Encodes a full octet (8 values) of a block of L values at width b > 0.
+// Plane bytes of the first 7 values of an octet (the last octet of a
+// fastMode-2 unit's block, which holds 8 * SPB - 1 values): 7 * nb bytes,
+// element-major little-endian, nothing written past them.
+__device__ __forceinline__ void store_planes7(uint8_t *p1, uint32_t nb, const uint32_t v[8])
+{
+    if (nb == 1) {
+        const uint64_t w = (uint64_t)(v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24)) |
+                           ((uint64_t)(v[4] | (v[5] << 8) | (v[6] << 16)) << 32);
+        lds_st_bytes(p1, w, 7);
+    } else if (nb == 2) {
+        lds_st64(p1, v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+        lds_st_bytes(p1 + 8, (uint64_t)(v[4] | (v[5] << 16)) | ((uint64_t)v[6] << 32), 6);
+    } else if (nb == 3) {
+        lds_st128(p1, v[0] | (v[1] << 24), (v[1] >> 8) | (v[2] << 16), (v[2] >> 16) | (v[3] << 8),
+                  v[4] | (v[5] << 24));
+        lds_st_bytes(p1 + 16, (uint64_t)((v[5] >> 8) | (v[6] << 16)) | ((uint64_t)(v[6] >> 16) << 32),
+                     5);
+    } else if (nb == 4) {
+        lds_st128(p1, v[0], v[1], v[2], v[3]);
+        lds_st64(p1 + 16, v[4], v[5]);
+        lds_st32(p1 + 24, v[6]);
+    }
+}
+
+// Encodes a full octet (8 values) of a block of L values at width b > 0;
+// SEVEN: the block's last octet, of 7 values (mg[7] == 0, sign bit 0).
+template <bool SEVEN = false>
 __device__ __forceinline__ void encode_octet8(uint8_t *ob, uint32_t ib, uint32_t o, uint32_t L,
                                               uint32_t b, const uint32_t mg[8], uint32_t sgn)
 {
@@ -630,7 +656,12 @@ __device__ __forceinline__ void encode_octet8(uint8_t *ob, uint32_t ib, uint32_t
     ob[ib + 1 + o] = (uint8_t)sgn;
     const uint32_t nb = b >> 3, r = b & 7;
     uint8_t *p1 = ob + ib + 1 + S8 + 8 * o * nb;
-    if (nb == 1) {
+    if (SEVEN) {
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = mg[i] >> r;
+        store_planes7(p1, nb, v);
+    } else if (nb == 1) {
         uint32_t v[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) v[i] = mg[i] >> r;
@@ -838,6 +869,111 @@ __device__ __forceinline__ void compress_simple_fast(
     }
 }
 
+// Fast path of k_compress for a fastMode-2 tile of whole units (unit_tile)
+// when SPB = B / 8 is a power of two up to 32.  A unit is B values: its
+// first value's q (the head) and a block of the other B - 1 deltas.  Fast
+// round r is K = 256 / SPB units (2048 values); thread tid owns values
+// r * 2048 + 8 * tid + [0, 8), octet o = tid % SPB of unit tid / SPB.  For
+// the block's layout the thread encodes the deltas of unit values 8o + 1 ..
+// 8o + 8 (block indices 8o .. 8o + 7), taking value 8o + 8 from the next
+// lane (DPP wave_shl:1; the unit's last lane has 7).  A tile holds at most
+// 4096 values, whose stream bytes always fit in ob, so every round is
+// encoded at once and the tile's total is published before any byte is.
+template <typename V, int SPB, bool INDEX>
+__device__ __forceinline__ void compress_units_fast(
+    const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob, uint32_t *s_wtot,
+    uint32_t *s_ioff, uint64_t *status, uint64_t tile, uint64_t &run)
+{
+    constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
+    constexpr int kRounds = kMaxRounds / 2;
+    static_assert(kRounds % 2 == 0, "the size scan packs rounds in pairs");
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
+    const uint32_t nu = nti / 2;  // units in the tile
+    const bool last = o + 1 == SPB;
+
+    V va[kRounds][8];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) va[r][i] = (V)0;
+        if (r * K + d < nu)
+            __builtin_memcpy(&va[r][0], x + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid,
+                             8 * sizeof(V));
+    }
+    uint32_t mg[kRounds][8], sg[kRounds], bw[kRounds], sz[kRounds];
+    int32_t hq[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        int32_t q[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) q[i] = quantize(va[r][i], inv);
+        const int32_t q8 = __builtin_amdgcn_update_dpp(0, q[0], 0x130, 0xF, 0xF, false);
+        hq[r] = q[0];
+        uint32_t omax = 0, sgn = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool keep = k < 7 || !last;
+            const int32_t dd = (int32_t)((uint32_t)(k < 7 ? q[k + 1] : q8) - (uint32_t)q[k]);
+            const uint32_t mag = keep ? (uint32_t)max(dd, (int32_t)(0u - (uint32_t)dd)) : 0u;
+            mg[r][k] = mag;
+            sgn = (sgn << 1) | (keep ? (uint32_t)dd >> 31 : 0u);
+            omax = max(omax, mag);
+        }
+        sg[r] = sgn;
+        const bool live = r * K + d < nu;
+        omax = item_max<SPB>(omax);
+        bw[r] = live ? bit_width(omax) : 0u;
+        // 4 head bytes + block_bytes(b, 8 * SPB - 1) = 1 + SPB * (b + 1) - b / 8.
+        sz[r] = live ? 4u + (bw[r] ? 1u + SPB * (bw[r] + 1u) - (bw[r] >> 3) : 1u) : 0u;
+    }
+    uint32_t ex[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; r += 2) {
+        const uint32_t v = o == 0 ? (sz[r] | (sz[r + 1] << 16)) : 0u;
+        const uint32_t incl = wave_incl_scan_u32(v, lane);
+        ex[r] = (incl & 0xFFFFu) - sz[r];
+        ex[r + 1] = (incl >> 16) - sz[r + 1];
+        const uint32_t wt = __builtin_amdgcn_readlane(incl, 63);
+        if (lane == 0) {
+            s_wtot[r * 4 + wv] = wt & 0xFFFFu;
+            s_wtot[(r + 1) * 4 + wv] = wt >> 16;
+        }
+    }
+    __syncthreads();
+    uint32_t off[kRounds];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        uint32_t before = 0, rt = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t t = s_wtot[r * 4 + w];
+            before += w < wv ? t : 0u;
+            rt += t;
+        }
+        off[r] = acc + before + ex[r];
+        acc += rt;
+    }
+    run = acc;
+    if (tid < 64) lookback_publish(status, tile, 0, acc);
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        if (r * K + d >= nu) continue;
+        if (o == 0) {
+            if (INDEX) s_ioff[2 * (r * K + d)] = off[r];  // the unit's head item
+            lds_st32(ob + off[r], (uint32_t)hq[r]);
+            ob[off[r] + 4] = (uint8_t)bw[r];
+        }
+        if (bw[r]) {
+            if (last)
+                encode_octet8<true>(ob, off[r] + 4, o, B - 1, bw[r], mg[r], sg[r]);
+            else
+                encode_octet8(ob, off[r] + 4, o, B - 1, bw[r], mg[r], sg[r]);
+        }
+    }
+}
+
 // A tile is g.R rounds of K items.  Each thread owns one octet per round and
 // loads it straight from HBM one round ahead of its use.  Rounds are encoded
 // into the LDS buffer ob (kOutCap bytes) as long as they fit; the first round
@@ -903,6 +1039,7 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     const uint64_t j_first = cj;
     const uint32_t nti_all = (uint32_t)(tIN - tI0);
     const bool simple = simple_tile(g, ct, cj, nti_all);
+    const bool units = SPB && unit_tile(g, ct, cj, nti_all);
     // The fast path writes the carry from data it loads anyway.
     if (INDEX && tid == 0 && !(SPB && simple))
         idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
@@ -918,6 +1055,16 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
                                                      s_wtot, s_rbase, s_ioff, status, tile, run,
                                                      nbuf, idx_carry);
         published = true;
+    } else if (units) {
+        compress_units_fast<V, (SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
+                                                      s_wtot, s_ioff, status, tile, run);
+        nbuf = g.R;  // the whole tile is in ob
+        published = true;
+        if (cj == 0 && tid == 0) {  // the chunk's first unit: its offset goes in the header
+            s_head_t[0] = ct;
+            s_head_off[0] = 0;
+            s_nheads = 1;
+        }
     } else {
         // General tiles (chunk heads, short blocks): every round's octet is
         // loaded before the first is used, so the tile pays one memory
@@ -1722,6 +1869,88 @@ __device__ __forceinline__ bool decode_simple_fast(
     return true;
 }
 
+// Fast path of k_decode for a fastMode-2 tile of whole units (unit_tile),
+// the mirror of compress_units_fast: the tile's bytes go to LDS once; thread
+// tid decodes block indices 8o .. 8o + 7 of its unit (o = tid % SPB; the
+// unit's last lane has 7), a scan within the unit's SPB lanes gives the
+// deltas before them, and the thread writes unit values 8o .. 8o + 7: value
+// 8o is the head plus those deltas, value 8o + k the same plus its own first
+// k deltas.  Units restart the Lorenzo chain, so no carry or look-back is
+// needed.  Returns false, before any side effect, for the field's last tile
+// (its end is not in the positions); the caller then takes the general path.
+template <typename V, int SPB>
+__device__ __forceinline__ bool decode_units_fast(
+    const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
+    const uint64_t *__restrict__ blkpos, V *__restrict__ out, uint32_t *ctl, uint64_t tIN,
+    uint32_t ct, uint64_t cj, uint32_t nti, V eb, uint8_t *buf, int *s_bad)
+{
+    constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
+    constexpr int kRounds = kMaxRounds / 2;
+    const int tid = threadIdx.x, lane = __lane_id();
+    const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
+    const uint32_t nu = nti / 2;
+    if (tIN >= g.nitems) return false;
+    const uint64_t pb = (uint64_t)ct * g.nb0 + cj / 2;  // blkpos index of the tile's first unit
+    uint64_t pos[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) pos[r] = (r * K + d < nu) ? blkpos[pb + r * K + d] : 0;
+    const uint64_t P0 = blkpos[pb];
+    uint32_t t1;
+    uint64_t j1;
+    locate(g, tIN, t1, j1);
+    // The tile ends where the next unit (or the next chunk) starts.
+    const uint64_t P1 = j1 ? blkpos[(uint64_t)t1 * g.nb0 + (j1 >> 1)]
+                           : g.body + read_u64_le4(st + g.es + 8ull * t1);
+    if (tid == 0) *s_bad = 0;
+    const uint64_t tbase = P0 & ~15ull;
+    bool bad = !(P0 < P1 && P1 <= len && P1 - tbase <= (uint64_t)kDecTileBytes);
+    if (!bad) load_bytes(st, len, P0, P1, buf);
+    __syncthreads();
+    const uint32_t lend = bad ? 0u : (uint32_t)(P1 - tbase);
+    uint32_t dl[kRounds][8];
+    int32_t hq[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        hq[r] = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dl[r][i] = 0;
+        if (r * K + d < nu && !bad) {
+            if (pos[r] < P0 || pos[r] + 5 > P1) {
+                bad = true;
+            } else {
+                const uint32_t lp = (uint32_t)(pos[r] - tbase);
+                uint32_t hv, h;
+                __builtin_memcpy(&hv, buf + lp, 4);
+                hq[r] = (int32_t)hv;
+                decode_slot(B - 1, false, o, buf, lp + 4, lend, dl[r], h, bad);
+            }
+        }
+    }
+    if (bad) *s_bad = 1;
+    __syncthreads();
+    if (*s_bad) {
+        if (tid == 0) raise_error(ctl, kErrStream);
+        return true;
+    }
+    const uint64_t Et = item_elem(g, ct, cj);
+    const int gs = lane - (int)o;  // the unit's first lane
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const uint32_t v = dl[r][7];
+        const uint32_t ex = wave_incl_scan_u32(v, lane) - v;
+        const uint32_t before = ex - (uint32_t)__shfl((int)ex, gs);  // deltas of the unit's earlier lanes
+        const uint32_t q0 = (uint32_t)__shfl(hq[r], gs) + before;
+        if (r * K + d < nu) {
+            V y[8];
+            y[0] = (V)(int32_t)q0 * eb;
+#pragma unroll
+            for (int i = 1; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i - 1]) * eb;
+            __builtin_memcpy(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y, 8 * sizeof(V));
+        }
+    }
+    return true;
+}
+
 // Decodes tiles of g.R rounds of K items.  The item positions of the tile
 // (chunk heads from the header, blocks from the parse) and, when they fit,
 // all its stream bytes are loaded once.  Pass 1 unpacks every round to get
@@ -1770,6 +1999,10 @@ __device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const 
         decode_simple_fast<V, (SPB ? SPB : 1), INDEXED>(st, len, g, blkpos, idx_carry, out, status,
                                                      ctl, tile, tIN, t0, j0, nti, eb, buf,
                                                      L.s_wsum, L.s_lb, &s_bad))
+        return;
+    if (SPB && unit_tile(g, t0, j0, nti) &&
+        decode_units_fast<V, (SPB ? SPB : 1)>(st, len, g, blkpos, out, ctl, tIN, t0, j0, nti, eb,
+                                              buf, &s_bad))
         return;
 
     // Item positions of the whole tile.

@@ -40,6 +40,26 @@ static void check_compress_tile(const Geom &g, uint64_t tile, std::vector<uint8_
     locate(g, tI0, ct, cj);
     const uint32_t nti = (uint32_t)(tIN - tI0);
     const bool simple = simple_tile(g, ct, cj, nti);
+    if (unit_tile(g, ct, cj, nti) && g.spb <= 32 && (g.spb & (g.spb - 1)) == 0) {
+        // The fastMode-2 unit fast path (instantiated for spb a power of two up
+        // to 32): thread tid of fast round r loads the 8 values
+        // Et + r * 2048 + 8 * tid + [0, 8) of unit r * K + tid / spb.
+        CHECK(!simple, "a tile both simple and of units");
+        const uint64_t Et = item_elem(g, ct, cj), lo = chunk_lo(g, ct);
+        CHECK(Et == lo + (cj / 2) * g.B, "unit tile start");
+        const uint32_t nu = nti / 2;
+        CHECK((uint64_t)nu <= (uint64_t)g.K * (kMaxRounds / 2), "unit tile holds more than its fast rounds");
+        for (uint32_t r = 0; r < (uint32_t)kMaxRounds / 2; ++r)
+            for (uint32_t tid = 0; tid < (uint32_t)kWG; ++tid) {
+                const uint32_t d = tid / g.spb;
+                if (r * g.K + d >= nu) continue;
+                const uint64_t e = Et + (uint64_t)r * 2048u + 8u * tid;
+                const uint64_t u = (e - lo) / g.B;  // unit of the octet within the chunk
+                CHECK(e + 8 <= g.n && e + 8 <= lo + chunk_len(g, ct), "unit octet past its chunk");
+                CHECK((e + 7 - lo) / g.B == u && u == cj / 2 + r * g.K + d,
+                      "unit octet outside its unit");
+            }
+    }
     uint32_t rt = ct;
     uint64_t rj = cj;
     for (uint32_t r = 0; r < g.R; ++r) {
