@@ -238,6 +238,23 @@ inline TileOrder general_tiles(const Geom &g, bool with_last = false)
 }
 
 // The tile that workgroup b runs.
+// ---- the pipelined decode with the side index ----
+// Workgroup w of a grid of G decodes tiles w, w + G, ...; it passes over at
+// most kPipeMaxSteps of them (its list of deferred tiles has that many bits).
+constexpr uint32_t kPipeMaxSteps = 4096;
+
+// Grid of the pipelined decode: the resident workgroups, at least one per
+// general tile (each decodes one), and enough that no workgroup passes over
+// more than kPipeMaxSteps tiles; never more than the tiles.
+inline uint64_t pipe_grid_size(const Geom &g, const TileOrder &ord, uint32_t resident)
+{
+    uint64_t G = g.ntiles < resident ? g.ntiles : resident;
+    const uint64_t need = (g.ntiles + kPipeMaxSteps - 1) / kPipeMaxSteps;
+    if (G < need) G = need;
+    if (G < ord.n) G = ord.n;
+    return G;
+}
+
 SZP_HD uint64_t tile_of_block(const TileOrder &o, uint64_t b)
 {
     if (o.n == 0) return b;

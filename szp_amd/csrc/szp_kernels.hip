@@ -1762,6 +1762,9 @@ __device__ __forceinline__ bool decode_simple_fast(
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
     if (tIN >= g.nitems) return false;
+    // The side index's carry is loaded with the positions, not after the
+    // unpack, so that its latency is not added to the tile's.
+    const uint32_t icarry = INDEXED ? idx_carry[tile] : 0u;
     const uint64_t pb = (uint64_t)ct * g.nb0 + (cj - 1);  // blkpos index of the first block
     uint64_t pos[kMaxRounds];
 #pragma unroll
@@ -1843,7 +1846,7 @@ __device__ __forceinline__ bool decode_simple_fast(
     }
     uint32_t carry;
     if (INDEXED) {
-        carry = idx_carry[tile];
+        carry = icarry;
     } else {
         // The tile holds no chunk head: its aggregate is a plain delta sum.
         if (tid < 64) lookback_publish(status, tile, 0, agg);
@@ -1975,7 +1978,7 @@ struct DecodeLds {
 // One decode tile (all threads of the workgroup call it; every exit is
 // uniform).
 template <typename V, bool INDEXED, int SPB>
-__device__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
+__device__ __forceinline__ void decode_tile(const uint8_t *__restrict__ st, uint64_t len, const Geom &g,
                             const uint64_t *__restrict__ blkpos,
                             const uint32_t *__restrict__ idx_carry, V *__restrict__ out,
                             uint64_t *__restrict__ status, uint32_t *__restrict__ ctl,
@@ -2176,6 +2179,239 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
         return;
     }
     decode_tile<V, INDEXED, SPB>(st, len, g, blkpos, idx_carry, out, status, ctl, tile, L);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent, software-pipelined decode with the side index
+// ---------------------------------------------------------------------------
+//
+// With the side index a tile needs nothing from any other tile, so one
+// workgroup can decode tiles w, w + G, w + 2G, ... (G = gridDim.x) in a
+// pipeline: while it unpacks tile c from LDS, the stream bytes, block
+// positions and carry of tile c + G are already in flight into registers,
+// and the byte range of tile c + 2G is being read.  A tile's two dependent
+// memory latencies (positions, then bytes) thus overlap the previous tile's
+// work instead of adding to it.  Simple tiles (full blocks of one chunk, B/8 a
+// power of two up to 32) whose bytes fit kPipeBytes go through the pipeline;
+// every other tile (chunk heads, the field's last tile, barely compressible
+// tiles) is decoded in place by decode_tile.  No tile waits for another, so
+// any grid size and any residency are correct.
+
+constexpr int kPipeVec = 4;                                 // 16-byte stream vectors per thread
+constexpr uint64_t kPipeBytes = 16ull * kWG * kPipeVec;     // a pipelined tile's byte span
+static_assert(kPipeBytes + 32 <= (uint64_t)kDecTileBytes + 64, "the tile buffer holds a pipelined tile");
+
+// One tile of the pipeline; uniform across the workgroup.
+struct PipeTile {
+    uint64_t tile;    // ~0 past the last tile
+    uint64_t P0, P1;  // stream bytes of a simple tile: its first block .. the next item
+    uint64_t pb;      // blkpos index of its first block
+    uint64_t Et;      // its first element
+    uint32_t nti;
+    bool simple;      // simple, and not the field's last tile
+};
+
+// Locates tile `tile` and, for a simple tile, issues the loads of its byte
+// range (used one pipeline step later).
+__device__ __forceinline__ PipeTile pipe_plan(const uint8_t *__restrict__ st, const Geom &g,
+                                              const uint64_t *__restrict__ blkpos, uint64_t tile)
+{
+    PipeTile p;
+    p.tile = tile < g.ntiles ? tile : ~0ull;
+    p.P0 = p.P1 = p.pb = p.Et = 0;
+    p.nti = 0;
+    p.simple = false;
+    if (p.tile == ~0ull) return p;
+    const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
+    const uint64_t tIN = min(tI0 + (uint64_t)g.K * g.R, g.nitems);
+    uint32_t ct;
+    uint64_t cj;
+    locate(g, tI0, ct, cj);
+    p.nti = (uint32_t)(tIN - tI0);
+    if (!simple_tile(g, ct, cj, p.nti) || tIN >= g.nitems) return p;
+    p.simple = true;
+    p.pb = (uint64_t)ct * g.nb0 + (cj - 1);
+    p.Et = item_elem(g, ct, cj);
+    p.P0 = blkpos[p.pb];
+    uint32_t t1;
+    uint64_t j1;
+    locate(g, tIN, t1, j1);
+    p.P1 = j1 ? blkpos[(uint64_t)t1 * g.nb0 + (j1 - 1)] : g.body + read_u64_le4(st + g.es + 8ull * t1);
+    return p;
+}
+
+// Issues the loads of a planned simple tile: its stream bytes (thread tid
+// holds the 16-byte vectors tid + kWG * v of the span from P0 rounded down
+// to 16), the positions of its slots' blocks and its carry.  Returns false
+// (no loads) when the tile goes to decode_tile instead.
+template <int SPB>
+__device__ __forceinline__ bool pipe_issue(const uint8_t *__restrict__ st, uint64_t len,
+                                           const uint64_t *__restrict__ blkpos,
+                                           const uint32_t *__restrict__ idx_carry,
+                                           const PipeTile &p, uint4 (&rb)[kPipeVec],
+                                           uint64_t (&pos)[kMaxRounds], uint32_t &carry)
+{
+    constexpr uint32_t K = 256 / SPB;
+    if (p.tile == ~0ull || !p.simple) return false;
+    const uint64_t a0 = p.P0 & ~15ull;
+    if (!(p.P0 < p.P1 && p.P1 <= len && p.P1 - a0 <= kPipeBytes)) return false;
+    const uint32_t tid = threadIdx.x, d = tid / SPB;
+#pragma unroll
+    for (int v = 0; v < kPipeVec; ++v) {
+        const uint64_t q = a0 + 16ull * (tid + (uint32_t)kWG * v);
+        rb[v] = make_uint4(0, 0, 0, 0);
+        if (q < p.P1) {
+            if (q + 16 <= len) {
+                rb[v] = *reinterpret_cast<const uint4 *>(st + q);
+            } else {
+                uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    if (q + k < len) w[k >> 2] |= (uint32_t)st[q + k] << (8 * (k & 3));
+                rb[v] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r)
+        pos[r] = (r * K + d < p.nti) ? blkpos[p.pb + r * K + d] : 0;
+    carry = idx_carry[p.tile];
+    return true;
+}
+
+// Unpacks a pipelined tile whose bytes and slot positions are in L and
+// stores its values (the second half of decode_simple_fast).
+template <typename V, int SPB>
+__device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V eb,
+                                            V *__restrict__ out, uint32_t *__restrict__ ctl,
+                                            DecodeLds &L)
+{
+    constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
+    const uint32_t lend = (uint32_t)(p.P1 - (p.P0 & ~15ull));
+    bool bad = false;
+    uint32_t dl[kMaxRounds][8];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dl[r][i] = 0;
+        if (r * K + d < p.nti) {
+            const uint32_t lp = L.s_pos[r * K + d];
+            uint32_t h;
+            if (lp == ~0u)
+                bad = true;
+            else
+                decode_slot(B, false, o, L.buf, lp, lend, dl[r], h, bad);
+        }
+    }
+    if (bad) raise_error(ctl, kErrStream);
+    uint32_t ex[kMaxRounds];
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        const uint32_t v = dl[r][7];
+        const uint32_t incl = wave_incl_scan_u32(v, lane);
+        ex[r] = incl - v;
+        const uint32_t wt = __builtin_amdgcn_readlane(incl, 63);
+        if (lane == 0) L.s_wsum[r * 4 + wv] = wt;
+    }
+    __syncthreads();
+    uint32_t base = carry;
+#pragma unroll
+    for (int r = 0; r < kMaxRounds; ++r) {
+        uint32_t before = base;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) before += w < wv ? L.s_wsum[r * 4 + w] : 0u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) base += L.s_wsum[r * 4 + w];
+        if (r * K + d < p.nti) {
+            const uint32_t q0 = before + ex[r];
+            V y[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
+            __builtin_memcpy(out + p.Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y, 8 * sizeof(V));
+        }
+    }
+}
+
+// Tiles the pipeline passes over are decoded by decode_tile after it ends,
+// with no pipeline state live: the general tiles (ord: chunk heads, short
+// last blocks and the field's last tile, one per workgroup) and the simple
+// tiles whose bytes exceed kPipeBytes (each workgroup marks its own in an
+// LDS bit list).
+// At most 128 VGPRs: four workgroups per CU, each with one tile in flight
+// and one being unpacked.
+template <typename V, int SPB>
+__global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restrict__ st, uint64_t len,
+                                                     Geom g, const uint64_t *__restrict__ blkpos,
+                                                     const uint32_t *__restrict__ idx_carry,
+                                                     V *__restrict__ out, uint32_t *__restrict__ ctl,
+                                                     TileOrder ord)
+{
+    constexpr uint32_t K = 256 / SPB;
+    __shared__ DecodeLds L;
+    __shared__ uint32_t s_defer[kPipeMaxSteps / 32];
+    const uint32_t tid = threadIdx.x, d = tid / SPB, o = tid % SPB;
+    const uint64_t G = gridDim.x;
+    const V eb = get_eb<V>(st);
+    for (uint32_t k = tid; k < kPipeMaxSteps / 32; k += kWG) s_defer[k] = 0;
+    PipeTile cur = pipe_plan(st, g, blkpos, blockIdx.x);
+    PipeTile nxt = pipe_plan(st, g, blkpos, blockIdx.x + G);
+    uint4 rb[kPipeVec];
+    uint64_t pos[kMaxRounds];
+    uint32_t carry = 0;
+    bool have = pipe_issue<SPB>(st, len, blkpos, idx_carry, cur, rb, pos, carry);
+    __syncthreads();
+    for (uint32_t step = 0; cur.tile != ~0ull; ++step) {
+        const PipeTile n2 = pipe_plan(st, g, blkpos, nxt.tile == ~0ull ? ~0ull : nxt.tile + G);
+        const uint32_t ccur = carry;
+        if (have) {
+            // This tile's bytes and slot positions go to LDS ...
+            const uint64_t a0 = cur.P0 & ~15ull;
+#pragma unroll
+            for (int v = 0; v < kPipeVec; ++v) {
+                const uint32_t e = 16u * (tid + (uint32_t)kWG * v);
+                if (a0 + e < cur.P1) *reinterpret_cast<uint4 *>(L.buf + e) = rb[v];
+            }
+            if (o == 0) {
+#pragma unroll
+                for (int r = 0; r < kMaxRounds; ++r)
+                    if (r * K + d < cur.nti)
+                        L.s_pos[r * K + d] = (pos[r] >= cur.P0 && pos[r] < cur.P1)
+                                                 ? (uint32_t)(pos[r] - a0)
+                                                 : ~0u;
+            }
+        } else if (cur.simple && tid == 0) {
+            s_defer[step >> 5] |= 1u << (step & 31);
+        }
+        __syncthreads();
+        // ... the next tile's loads go out, then this tile is unpacked.
+        const bool have_n = pipe_issue<SPB>(st, len, blkpos, idx_carry, nxt, rb, pos, carry);
+        if (have) pipe_decode<V, SPB>(cur, ccur, eb, out, ctl, L);
+        __syncthreads();  // L is reused by the next tile
+        have = have_n;
+        cur = nxt;
+        nxt = n2;
+    }
+    // The workgroup's general tile, then its deferred ones (one call site,
+    // so that decode_tile is inlined).
+    bool gen = blockIdx.x < ord.n;
+    uint32_t w = 0, bits = s_defer[0];
+    while (true) {
+        uint64_t t;
+        if (gen) {
+            t = ord.t[blockIdx.x];
+            gen = false;
+        } else {
+            while (!bits && ++w < kPipeMaxSteps / 32) bits = s_defer[w];
+            if (!bits) break;
+            const uint32_t k = (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1;
+            t = blockIdx.x + (uint64_t)(32 * w + k) * G;
+        }
+        decode_tile<V, true, SPB>(st, len, g, blkpos, idx_carry, out, nullptr, ctl, t, L);
+        __syncthreads();
+    }
 }
 
 // fastMode 2 with B = 1: every unit is one value, which the compressor
@@ -2802,6 +3038,25 @@ int launch_decompress(int dtype, const uint8_t *d_stream, uint64_t len, uint64_t
     return kArg;
 }
 
+// Workgroups of k_decode_pipe<V, SPB> the device holds at once (occupancy
+// times CUs), found once per process; any grid size is correct for it, this
+// one keeps every CU busy without queued workgroups.
+template <typename V, int SPB>
+static uint32_t pipe_grid()
+{
+    static const uint32_t grid = [] {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_decode_pipe<V, SPB>, kWG, 0) !=
+                hipSuccess ||
+            cus <= 0 || per_cu <= 0)
+            return 1024u;
+        return (uint32_t)(cus * per_cu);
+    }();
+    return grid;
+}
+
 template <typename V>
 static int launch_decompress_indexed_t(int mode, const uint8_t *d_stream, uint64_t len,
                                        const void *d_index, uint64_t idx_bytes, uint64_t n, int B,
@@ -2830,6 +3085,29 @@ static int launch_decompress_indexed_t(int mode, const uint8_t *d_stream, uint64
     uint64_t *ipos;
     uint32_t *icarry;
     index_layout(g, reinterpret_cast<uint8_t *>(const_cast<void *>(d_index)), &ipos, &icarry);
+    const uint32_t spb = (g.B & 7) ? 0u : g.B / 8;
+    // The pipelined kernel needs every general tile in the list (at most
+    // kMaxGenTiles of them); the last tile is always one.
+    const TileOrder ord = general_tiles(g, true);
+    if (mode == 1 && spb && spb <= 32 && !(spb & (spb - 1)) && ord.n) {
+        switch (spb) {
+#define SZP_PIPE_CASE(S)                                                                         \
+    case S: {                                                                                    \
+        const dim3 grid((uint32_t)pipe_grid_size(g, ord, pipe_grid<V, S>()));                    \
+        SZP_LAUNCH("k_decode_indexed", (k_decode_pipe<V, S>), grid, dim3(kWG), 0, stream,        \
+                   d_stream, len, g, ipos, icarry, d_out, ctl, ord);                             \
+        break;                                                                                   \
+    }
+            SZP_PIPE_CASE(1)
+            SZP_PIPE_CASE(2)
+            SZP_PIPE_CASE(4)
+            SZP_PIPE_CASE(8)
+            SZP_PIPE_CASE(16)
+            SZP_PIPE_CASE(32)
+#undef SZP_PIPE_CASE
+        }
+        return hipGetLastError() == hipSuccess ? kOk : kHip;
+    }
     launch_k_decode<V, true>("k_decode_indexed", g, stream, d_stream, len, ipos, icarry, d_out,
                              nullptr, ctl);
     return hipGetLastError() == hipSuccess ? kOk : kHip;

@@ -168,6 +168,17 @@ static void check_order(const Geom &g, bool with_last)
     }
     // Without a list, every tile takes blockIdx order.
     if (!in_use) CHECK(tile_of_block(o, 5) == 5, "identity order");
+    // The pipelined decode's grid: workgroup w takes tiles w, w + G, ... and
+    // the general tile o.t[w] for w < o.n.
+    if (with_last && in_use && g.ntiles) {
+        for (uint32_t resident : {1u, 3u, 1024u, 1u << 20}) {
+            const uint64_t G = pipe_grid_size(g, o, resident);
+            CHECK(G >= 1 && G <= g.ntiles, "pipe grid %llu for %llu tiles", (unsigned long long)G,
+                  (unsigned long long)g.ntiles);
+            CHECK(G >= o.n, "pipe grid below the general tiles");
+            CHECK((g.ntiles + G - 1) / G <= kPipeMaxSteps, "pipe steps exceed the deferral bits");
+        }
+    }
 }
 
 static void check_geometry(uint64_t n, int B, int T, bool full, uint32_t mode = 1)
