@@ -185,13 +185,18 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane)
+// Inclusive wave64 prefix sum in six DPP steps: row_shr 1, 2, 4, 8 scan each
+// row of 16 lanes (a lane whose source lies before its row keeps the 0 it
+// was given), then row_bcast:15 adds row 0's total to row 1 and row 2's to
+// row 3, and row_bcast:31 adds the total of rows 0-1 (lane 31) to rows 2-3.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int /*lane*/)
 {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d);
-        if (lane >= d) v += t;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return v;
 }
 
@@ -1603,7 +1608,9 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
 // run up to 32 bytes past it, inside the buffer's slack).  dl[i] = inclusive
 // prefix of the slot's deltas, or the absolute head value in dl[0] for a head
 // slot (h = 1).  Returns the number of values (0 for idle slots); sets bad on
-// an invalid width or a block that overruns the bytes present.
+// an invalid width or a block that overruns the bytes present.  FULL: the
+// caller guarantees a full octet (8 * o + 8 <= L), so no value is masked.
+template <bool FULL = false>
 __device__ __forceinline__ uint32_t decode_slot(uint32_t L, bool head, uint32_t o,
                                                 const uint8_t *buf, uint32_t lp, uint32_t lend,
                                                 uint32_t dl[8], uint32_t &h, bool &bad)
@@ -1670,10 +1677,12 @@ __device__ __forceinline__ uint32_t decode_slot(uint32_t L, bool head, uint32_t 
     uint32_t run = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const uint32_t low = r ? (uint32_t)(A >> (64 - r * (i + 1))) & mask : 0u;
+        // With r == 0 both A and mask are 0, so the shift amount (mod 64) is moot.
+        const uint32_t low = (uint32_t)(A >> ((64u - r * (i + 1)) & 63u)) & mask;
         const uint32_t mag = (v[i] << r) | low;
-        const bool neg = (sg >> (7 - i)) & 1u;
-        if ((uint32_t)i < m) run += neg ? 0u - mag : mag;
+        // s = ~0 when sign bit 7 - i is set; (mag ^ s) - s is then 0 - mag.
+        const uint32_t s = (uint32_t)((int32_t)(sg << (24 + i)) >> 31);
+        if (FULL || (uint32_t)i < m) run += (mag ^ s) - s;
         dl[i] = run;
     }
     return m;
@@ -1796,7 +1805,8 @@ __device__ __forceinline__ bool decode_simple_fast(
                 if (pos[r] < P0 || pos[r] >= P1)
                     bad = true;
                 else
-                    decode_slot(B, false, o, buf, (uint32_t)(pos[r] - tbase), lend, dl[r], h, bad);
+                    decode_slot<true>(B, false, o, buf, (uint32_t)(pos[r] - tbase), lend, dl[r], h,
+                                      bad);
             }
         }
     } else if (!bad) {
@@ -1820,7 +1830,7 @@ __device__ __forceinline__ bool decode_simple_fast(
                 if (pos[r] < Pr || pos[r] >= Pn)
                     bad = true;
                 else
-                    decode_slot(B, false, o, buf, (uint32_t)(pos[r] - rbase), (uint32_t)(Pn - rbase),
+                    decode_slot<true>(B, false, o, buf, (uint32_t)(pos[r] - rbase), (uint32_t)(Pn - rbase),
                                 dl[r], h, bad);
             }
         }
@@ -2302,7 +2312,7 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
             if (lp == ~0u)
                 bad = true;
             else
-                decode_slot(B, false, o, L.buf, lp, lend, dl[r], h, bad);
+                decode_slot<true>(B, false, o, L.buf, lp, lend, dl[r], h, bad);
         }
     }
     if (bad) raise_error(ctl, kErrStream);
