@@ -1615,23 +1615,26 @@ __device__ __forceinline__ uint32_t decode_slot(uint32_t L, bool head, uint32_t 
                                                 const uint8_t *buf, uint32_t lp, uint32_t lend,
                                                 uint32_t dl[8], uint32_t &h, bool &bad)
 {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) dl[i] = 0;
     h = 0;
-    if (head) {
-        if (o) return 0;
-        h = 1;
-        __builtin_memcpy(&dl[0], buf + lp, 4);
-        return 1;
+    if (!FULL) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dl[i] = 0;
+        if (head) {
+            if (o) return 0;
+            h = 1;
+            __builtin_memcpy(&dl[0], buf + lp, 4);
+            return 1;
+        }
+        if (8 * o >= L) return 0;
     }
-    if (8 * o >= L) return 0;
-    const uint32_t m = min(8u, L - 8 * o);
-    const uint32_t b = buf[lp];
-    if (b > 32 || lp + block_bytes(b, L) > lend) {
-        bad = true;
-        return m;
-    }
-    if (!b) return m;
+    const uint32_t m = FULL ? 8u : min(8u, L - 8 * o);
+    // A bad width or a block past the bytes present is reported and decoded
+    // as width 0; every path below writes all of dl (no early exit, so the
+    // compiler need not clear dl at each branch).
+    const uint32_t b0 = buf[lp];
+    const bool okb = b0 <= 32 && lp + block_bytes(b0, L) <= lend;
+    if (!okb) bad = true;
+    const uint32_t b = okb ? b0 : 0u;
     const uint32_t S8 = (L + 7) >> 3;
     const uint32_t sg = buf[lp + 1 + o];
     const uint32_t nb = b >> 3, r = b & 7;
@@ -1771,6 +1774,8 @@ __device__ __forceinline__ bool decode_simple_fast(
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
     if (tIN >= g.nitems) return false;
+    // Not the field's last tile, so the tile holds whole rounds (nti = K * R)
+    // and the round tests below are uniform.
     // The side index's carry is loaded with the positions, not after the
     // unpack, so that its latency is not added to the tile's.
     const uint32_t icarry = INDEXED ? idx_carry[tile] : 0u;
@@ -1778,7 +1783,7 @@ __device__ __forceinline__ bool decode_simple_fast(
     uint64_t pos[kMaxRounds];
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r)
-        pos[r] = (r * K + d < nti) ? blkpos[pb + r * K + d] : 0;
+        pos[r] = ((uint32_t)r * K < nti) ? blkpos[pb + r * K + d] : 0;
     const uint64_t P0 = blkpos[pb];
     uint32_t t1;
     uint64_t j1;
@@ -1800,7 +1805,7 @@ __device__ __forceinline__ bool decode_simple_fast(
         const uint32_t lend = (uint32_t)(P1 - tbase);
 #pragma unroll
         for (int r = 0; r < kMaxRounds; ++r) {
-            if (r * K + d < nti) {
+            if ((uint32_t)r * K < nti) {
                 uint32_t h;
                 if (pos[r] < P0 || pos[r] >= P1)
                     bad = true;
@@ -1825,7 +1830,7 @@ __device__ __forceinline__ bool decode_simple_fast(
             __syncthreads();
             if (!ok) {
                 bad = true;
-            } else if (r * K + d < nti) {
+            } else if ((uint32_t)r * K < nti) {
                 uint32_t h;
                 if (pos[r] < Pr || pos[r] >= Pn)
                     bad = true;
@@ -1871,7 +1876,7 @@ __device__ __forceinline__ bool decode_simple_fast(
         for (int w = 0; w < 4; ++w) before += w < wv ? s_wsum[r * 4 + w] : 0u;
 #pragma unroll
         for (int w = 0; w < 4; ++w) base += s_wsum[r * 4 + w];
-        if (r * K + d < nti) {
+        if ((uint32_t)r * K < nti) {
             const uint32_t q0 = before + ex[r];
             V y[8];
 #pragma unroll
@@ -2284,7 +2289,7 @@ __device__ __forceinline__ bool pipe_issue(const uint8_t *__restrict__ st, uint6
     }
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r)
-        pos[r] = (r * K + d < p.nti) ? blkpos[p.pb + r * K + d] : 0;
+        pos[r] = ((uint32_t)r * K < p.nti) ? blkpos[p.pb + r * K + d] : 0;
     carry = idx_carry[p.tile];
     return true;
 }
@@ -2300,19 +2305,20 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
     const uint32_t lend = (uint32_t)(p.P1 - (p.P0 & ~15ull));
+    // Every tile but the field's last holds whole rounds (nti = K * R), so
+    // the round tests below are uniform.
     bool bad = false;
     uint32_t dl[kMaxRounds][8];
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) dl[r][i] = 0;
-        if (r * K + d < p.nti) {
-            const uint32_t lp = L.s_pos[r * K + d];
+        if ((uint32_t)r * K < p.nti) {
+            const uint32_t lp0 = L.s_pos[r * K + d];
+            if (lp0 == ~0u) bad = true;
             uint32_t h;
-            if (lp == ~0u)
-                bad = true;
-            else
-                decode_slot<true>(B, false, o, L.buf, lp, lend, dl[r], h, bad);
+            decode_slot<true>(B, false, o, L.buf, lp0 == ~0u ? 0u : lp0, lend, dl[r], h, bad);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dl[r][i] = 0;
         }
     }
     if (bad) raise_error(ctl, kErrStream);
@@ -2334,7 +2340,7 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
         for (int w = 0; w < 4; ++w) before += w < wv ? L.s_wsum[r * 4 + w] : 0u;
 #pragma unroll
         for (int w = 0; w < 4; ++w) base += L.s_wsum[r * 4 + w];
-        if (r * K + d < p.nti) {
+        if ((uint32_t)r * K < p.nti) {
             const uint32_t q0 = before + ex[r];
             V y[8];
 #pragma unroll
@@ -2386,7 +2392,7 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
             if (o == 0) {
 #pragma unroll
                 for (int r = 0; r < kMaxRounds; ++r)
-                    if (r * K + d < cur.nti)
+                    if ((uint32_t)r * K < cur.nti)
                         L.s_pos[r * K + d] = (pos[r] >= cur.P0 && pos[r] < cur.P1)
                                                  ? (uint32_t)(pos[r] - a0)
                                                  : ~0u;
