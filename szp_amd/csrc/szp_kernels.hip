@@ -796,19 +796,23 @@ __device__ __forceinline__ void compress_simple_fast(
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
 
+    // Every slot loads unconditionally, so that all rounds' loads go out back
+    // to back (a conditional load whose result merges with a default value
+    // makes the compiler wait for it before the next round's loads).  Idle
+    // slots (past the tile's items) read the tile's first octet instead and
+    // their values are never used: their blocks are not live, and a live
+    // lane's predecessor lane is live.  Et >= 1 (the tile's first item is a
+    // block), so e >= 1.
     V va[kMaxRounds][8];
     V p0[kMaxRounds];
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) va[r][i] = (V)0;
-        p0[r] = (V)0;
-        if (r * K + d < nti) {
-            // Et >= 1 (the tile's first item is a block), so e >= 1.
-            const uint64_t e = Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid;
-            __builtin_memcpy(&va[r][0], x + e, 8 * sizeof(V));
-            if (lane == 0) p0[r] = x[e - 1];
-        }
+        const uint64_t e =
+            Et + ((r * K + d < nti) ? (uint64_t)r * 2048u + 8u * (uint32_t)tid : 0u);
+        __builtin_memcpy(&va[r][0], x + e, 8 * sizeof(V));
+        // Only lane 0 uses it; every lane loads it (from the lines it reads
+        // anyway) so that no branch splits the loads.
+        p0[r] = x[e - 1];
     }
 
     // Side index: the q of the element before the tile (thread 0 loaded it).
@@ -897,15 +901,14 @@ __device__ __forceinline__ void compress_units_fast(
     const uint32_t nu = nti / 2;  // units in the tile
     const bool last = o + 1 == SPB;
 
+    // Unconditional loads (idle slots read the tile's first octet; their
+    // units are not live), so that both rounds' loads go out back to back.
     V va[kRounds][8];
 #pragma unroll
-    for (int r = 0; r < kRounds; ++r) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) va[r][i] = (V)0;
-        if (r * K + d < nu)
-            __builtin_memcpy(&va[r][0], x + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid,
-                             8 * sizeof(V));
-    }
+    for (int r = 0; r < kRounds; ++r)
+        __builtin_memcpy(&va[r][0],
+                         x + Et + ((r * K + d < nu) ? (uint64_t)r * 2048u + 8u * (uint32_t)tid : 0u),
+                         8 * sizeof(V));
     uint32_t mg[kRounds][8], sg[kRounds], bw[kRounds], sz[kRounds];
     int32_t hq[kRounds];
 #pragma unroll
