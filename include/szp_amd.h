@@ -190,6 +190,9 @@ int szp_amd_bit_width(uint32_t maxMagnitude);
  * reading syncs those events and returns the per-kernel launch count and
  * summed milliseconds.  enable(1) and enable(0) both clear the totals. */
 void szp_amd_profile_enable(int enable);
+/* As szp_amd_profile_enable(1), timing only the launches of one kernel
+ * (e.g. "k_compress"), so that the events disturb the others least. */
+void szp_amd_profile_enable_only(const char *kernel);
 int szp_amd_profile_count(void);
 int szp_amd_profile_entry(int index, char *name, int nameCapacity, uint64_t *launches,
                           double *totalMs);

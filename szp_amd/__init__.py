@@ -105,6 +105,7 @@ def _load():
                                                u64, vp]),
         "szp_amd_stream_length_typed": (u64, [i32, i32, vp, u64, i32, i32]),
         "szp_amd_profile_enable": (None, [i32]),
+        "szp_amd_profile_enable_only": (None, [ctypes.c_char_p]),
         "szp_amd_profile_count": (i32, []),
         "szp_amd_profile_entry": (i32, [i32, ctypes.c_char_p, i32, ctypes.POINTER(u64),
                                         ctypes.POINTER(dbl)]),
@@ -126,9 +127,13 @@ def _last_error():
     return (lib.szp_amd_last_error() or b"").decode()
 
 
-def profile_enable(on=True):
-    """Starts (or stops) per-kernel event timing; both clear the totals."""
-    lib.szp_amd_profile_enable(1 if on else 0)
+def profile_enable(on=True, only=None):
+    """Starts (or stops) per-kernel event timing; both clear the totals.
+    only: time just the launches of this kernel name."""
+    if on and only:
+        lib.szp_amd_profile_enable_only(only.encode())
+    else:
+        lib.szp_amd_profile_enable(1 if on else 0)
 
 
 def profile_read():

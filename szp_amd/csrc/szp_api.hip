@@ -315,6 +315,8 @@ int szp_amd_bit_width(uint32_t maxMagnitude) { return bit_width_host(maxMagnitud
 
 void szp_amd_profile_enable(int enable) { profile_enable(enable); }
 
+void szp_amd_profile_enable_only(const char *kernel) { profile_enable(1, kernel); }
+
 int szp_amd_profile_count(void) { return profile_count(); }
 
 int szp_amd_profile_entry(int index, char *name, int nameCapacity, uint64_t *launches,

@@ -81,7 +81,7 @@ int batch_launch(const BatchPlan &plan, uint32_t ticket, uint32_t grid, hipStrea
 uint32_t *workspace_error_word(void *d_work);
 
 // Per-kernel event timing (off by default).
-void profile_enable(int enable);
+void profile_enable(int enable, const char *only = nullptr);
 int profile_count();
 int profile_entry(int i, char *name, int cap, uint64_t *launches, double *total_ms);
 

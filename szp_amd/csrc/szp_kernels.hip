@@ -2496,6 +2496,7 @@ struct Rec {
 
 std::mutex mu;
 bool on = false;
+std::string only;  // when not empty, only launches of this kernel are timed
 std::vector<Rec> recs;
 std::vector<hipEvent_t> pool;
 std::vector<std::pair<std::string, std::pair<uint64_t, double>>> totals;
@@ -2518,7 +2519,7 @@ struct Scope {
     Scope(const char *name, hipStream_t st) : s(st)
     {
         std::lock_guard<std::mutex> lk(mu);
-        if (!on) return;
+        if (!on || (!only.empty() && only != name)) return;
         r.name = name;
         r.a = take();
         r.b = take();
@@ -2715,12 +2716,13 @@ __global__ __launch_bounds__(kWG) void k_batch_decode(const BatchJob *__restrict
         hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);       \
     } while (0)
 
-void profile_enable(int enable)
+void profile_enable(int enable, const char *only)
 {
     std::lock_guard<std::mutex> lk(prof::mu);
     prof::collect_locked();
     prof::totals.clear();
     prof::on = enable != 0;
+    prof::only = only ? only : "";
 }
 
 int profile_count()
