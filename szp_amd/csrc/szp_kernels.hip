@@ -2292,7 +2292,7 @@ __device__ __forceinline__ bool pipe_issue(const uint8_t *__restrict__ st, uint6
     }
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r)
-        pos[r] = ((uint32_t)r * K < p.nti) ? blkpos[p.pb + r * K + d] : 0;
+        pos[r] = blkpos[p.pb + r * K + d];  // whole tiles of kMaxRounds rounds
     carry = idx_carry[p.tile];
     return true;
 }
@@ -2308,21 +2308,17 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
     const uint32_t lend = (uint32_t)(p.P1 - (p.P0 & ~15ull));
-    // Every tile but the field's last holds whole rounds (nti = K * R), so
-    // the round tests below are uniform.
+    // The pipeline runs only on fields whose tiles are kMaxRounds whole
+    // rounds (g.R == kMaxRounds), so every slot of every round is live and
+    // the number of stores after the next tile's loads is fixed.
     bool bad = false;
     uint32_t dl[kMaxRounds][8];
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
-        if ((uint32_t)r * K < p.nti) {
-            const uint32_t lp0 = L.s_pos[r * K + d];
-            if (lp0 == ~0u) bad = true;
-            uint32_t h;
-            decode_slot<true>(B, false, o, L.buf, lp0 == ~0u ? 0u : lp0, lend, dl[r], h, bad);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) dl[r][i] = 0;
-        }
+        const uint32_t lp0 = L.s_pos[r * K + d];
+        if (lp0 == ~0u) bad = true;
+        uint32_t h;
+        decode_slot<true>(B, false, o, L.buf, lp0 == ~0u ? 0u : lp0, lend, dl[r], h, bad);
     }
     if (bad) raise_error(ctl, kErrStream);
     uint32_t ex[kMaxRounds];
@@ -2343,13 +2339,11 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
         for (int w = 0; w < 4; ++w) before += w < wv ? L.s_wsum[r * 4 + w] : 0u;
 #pragma unroll
         for (int w = 0; w < 4; ++w) base += L.s_wsum[r * 4 + w];
-        if ((uint32_t)r * K < p.nti) {
-            const uint32_t q0 = before + ex[r];
-            V y[8];
+        const uint32_t q0 = before + ex[r];
+        V y[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
-            __builtin_memcpy(out + p.Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y, 8 * sizeof(V));
-        }
+        for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
+        __builtin_memcpy(out + p.Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y, 8 * sizeof(V));
     }
 }
 
@@ -2395,10 +2389,9 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
             if (o == 0) {
 #pragma unroll
                 for (int r = 0; r < kMaxRounds; ++r)
-                    if ((uint32_t)r * K < cur.nti)
-                        L.s_pos[r * K + d] = (pos[r] >= cur.P0 && pos[r] < cur.P1)
-                                                 ? (uint32_t)(pos[r] - a0)
-                                                 : ~0u;
+                    L.s_pos[r * K + d] = (pos[r] >= cur.P0 && pos[r] < cur.P1)
+                                             ? (uint32_t)(pos[r] - a0)
+                                             : ~0u;
             }
         } else if (cur.simple && tid == 0) {
             s_defer[step >> 5] |= 1u << (step & 31);
@@ -3108,9 +3101,10 @@ static int launch_decompress_indexed_t(int mode, const uint8_t *d_stream, uint64
     index_layout(g, reinterpret_cast<uint8_t *>(const_cast<void *>(d_index)), &ipos, &icarry);
     const uint32_t spb = (g.B & 7) ? 0u : g.B / 8;
     // The pipelined kernel needs every general tile in the list (at most
-    // kMaxGenTiles of them); the last tile is always one.
+    // kMaxGenTiles of them; the last tile is always one) and tiles of
+    // kMaxRounds whole rounds (chunks not too short).
     const TileOrder ord = general_tiles(g, true);
-    if (mode == 1 && spb && spb <= 32 && !(spb & (spb - 1)) && ord.n) {
+    if (mode == 1 && spb && spb <= 32 && !(spb & (spb - 1)) && ord.n && g.R == (uint32_t)kMaxRounds) {
         switch (spb) {
 #define SZP_PIPE_CASE(S)                                                                         \
     case S: {                                                                                    \
