@@ -21,6 +21,7 @@
 #include <stdint.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <mutex>
@@ -1609,9 +1610,10 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
 // Writes one thread's eight output values.  NT: non-temporal stores, so
 // that an output far larger than the memory-side cache does not leave it
 // full of dirty lines for the next pass over the field.  Measured on the
-// 512^3 field: the look-back decode 330 -> 286 us; the pipelined decode,
-// which waits for its stores before staging the next tile, 161 -> 218 us, so
-// it keeps ordinary stores.
+// 512^3 field: the look-back decode 330 -> 286 us for float (the float64
+// foreign-stream decode was slower with them: 1.28 -> 0.83 TB/s, so float64
+// keeps ordinary stores); the pipelined decode, which waits for its stores
+// before staging the next tile, 161 -> 218 us, so it keeps ordinary stores.
 template <bool NT, typename V>
 __device__ __forceinline__ void store_octet_out(V *p, const V y[8])
 {
@@ -1906,7 +1908,7 @@ __device__ __forceinline__ bool decode_simple_fast(
             V y[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
-            store_octet_out<true>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
+            store_octet_out<sizeof(V) == 4>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
         }
     }
     return true;
@@ -1988,7 +1990,7 @@ __device__ __forceinline__ bool decode_units_fast(
             y[0] = (V)(int32_t)q0 * eb;
 #pragma unroll
             for (int i = 1; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i - 1]) * eb;
-            store_octet_out<true>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
+            store_octet_out<sizeof(V) == 4>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
         }
     }
     return true;
@@ -3076,10 +3078,14 @@ int launch_decompress(int dtype, const uint8_t *d_stream, uint64_t len, uint64_t
 
 // Workgroups of k_decode_pipe<V, SPB> the device holds at once (occupancy
 // times CUs), found once per process; any grid size is correct for it, this
-// one keeps every CU busy without queued workgroups.
+// one keeps every CU busy without queued workgroups.  SZP_AMD_PIPE_GRID=g
+// (g > 0) caps it, so that a small field still runs many pipeline steps per
+// workgroup (a test and tuning knob).
 template <typename V, int SPB>
 static uint32_t pipe_grid()
 {
+    const char *cap = getenv("SZP_AMD_PIPE_GRID");
+    const long capv = cap ? strtol(cap, nullptr, 10) : 0;
     static const uint32_t grid = [] {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -3090,7 +3096,7 @@ static uint32_t pipe_grid()
             return 1024u;
         return (uint32_t)(cus * per_cu);
     }();
-    return grid;
+    return capv > 0 && (uint64_t)capv < grid ? (uint32_t)capv : grid;
 }
 
 template <typename V>

@@ -155,6 +155,14 @@ INDEXED_CASES = [
     ("walk", 40, 8, 16, 0, 0.01, 0.0),
     ("specials", 200000, 32, 4, 0, 0.5, 0.0),
     pytest.param(("uniform", 300000, 2048, 2, 0, 0.002, 0.0), marks=pytest.mark.skip(reason=FAULT_REASON)),
+    # Pipelined decode: tiles over 16 KiB of stream (decoded after the
+    # pipeline), B = 8 / 16 / 256, and more general tiles than the list holds
+    # (T = 80: the look-back-free tile kernel instead).
+    ("uniform", 1 << 20, 64, 8, 0, 1e-5, 0.0),
+    ("walk", 500000, 8, 4, 0, 0.01, 0.0),
+    ("waves", 3000017, 16, 31, 1, 0.0, 1e-3),
+    ("waves", 1 << 20, 256, 8, 1, 0.0, 1e-4),
+    ("waves", 1 << 20, 32, 80, 1, 0.0, 1e-3),
 ]
 
 
@@ -178,3 +186,35 @@ def test_side_index_roundtrip(szp, torch_mod, case):
     codec.check()
     ref = oracle_lib.decompress(want, n, B, T)
     assert np.array_equal(dec.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+PIPE_CASES = [
+    ("waves", (1 << 20) + 13, 128, 16, 1, 0.0, 1e-4),
+    ("uniform", 1 << 20, 64, 8, 0, 1e-5, 0.0),
+    ("walk", 500000, 8, 4, 0, 0.01, 0.0),
+    ("waves", 1 << 20, 256, 8, 1, 0.0, 1e-4),
+]
+
+
+@pytest.mark.parametrize("case", PIPE_CASES, ids=lambda c: "%s-n%d-B%d-T%d" % (c[0], c[1], c[2], c[3]))
+def test_pipelined_decode_many_steps(szp, torch_mod, case, monkeypatch):
+    """The pipelined decode with its grid capped (SZP_AMD_PIPE_GRID), so that
+    each workgroup runs many pipeline steps and defers several tiles: the
+    result is bit-identical to the oracle's decompression."""
+    torch = torch_mod
+    kind, n, B, T, mode, ab, rel = case
+    x = make_input(kind, n, 23 + n)
+    say("pipelined case", case)
+    want = oracle_lib.compress(x, mode, ab, rel, B, T)
+    codec = szp.DeviceCodec(block_size=B, threads=T)
+    xd = torch.from_numpy(x).cuda()
+    out, size_t, idx = codec.compress(xd, mode, ab, rel, index=True)
+    codec.check()
+    size = int(size_t.item())
+    assert np.array_equal(out[:size].cpu().numpy(), want)
+    ref = oracle_lib.decompress(want, n, B, T)
+    for cap in ("3", "13"):
+        monkeypatch.setenv("SZP_AMD_PIPE_GRID", cap)
+        dec = codec.decompress(out, size, n, index=idx)
+        codec.check()
+        assert np.array_equal(dec.cpu().numpy().view(np.uint32), ref.view(np.uint32)), cap
