@@ -372,9 +372,14 @@ __device__ __forceinline__ void vec_minmax(const uint8_t *p, double &mn, double 
 constexpr int kRangeUnroll = 4;
 
 // Segments are taken from the field's end backwards (workgroup b reads
-// segment gridDim.x - 1 - b): the field's first bytes, which the compress
-// kernel reads first, are the ones read last here and are still in the
-// memory-side cache (256 MB on MI355X) when it starts.
+// segment gridDim.x - 1 - b), with non-temporal loads.  Measured on a 512^3
+// float field: with ordinary loads the pass ran at 88 us after a compress
+// (the field's head, read last here and first by the compress kernel, was
+// still in the 256 MB memory-side cache) but at 134 us after a decode (the
+// cache then held the decode's dirty output, which had to be written back
+// first); with non-temporal loads it runs at 82 us after either, and the
+// compress kernel's loss of those cache hits (233 -> 255 us) is smaller than
+// the gain (compress + decompress step 0.530 -> 0.503 ms).
 template <typename V, bool VEC>
 __global__ __launch_bounds__(kWG) void k_range_partial(const V *__restrict__ x, uint64_t n,
                                                        uint64_t seg, RangePart<V> *part)
@@ -389,10 +394,11 @@ __global__ __launch_bounds__(kWG) void k_range_partial(const V *__restrict__ x, 
         constexpr uint32_t kStep = kPer * kWG * kRangeUnroll;
         const uint32_t full = len / kStep * kStep;
         for (uint32_t v = kPer * threadIdx.x; v < full; v += kStep) {
-            uint4 q[kRangeUnroll];
+            typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+            u4v q[kRangeUnroll];
 #pragma unroll
             for (int u = 0; u < kRangeUnroll; ++u)
-                q[u] = *reinterpret_cast<const uint4 *>(xs + v + kPer * kWG * u);
+                q[u] = __builtin_nontemporal_load(reinterpret_cast<const u4v *>(xs + v + kPer * kWG * u));
 #pragma unroll
             for (int u = 0; u < kRangeUnroll; ++u)
                 vec_minmax(reinterpret_cast<const uint8_t *>(&q[u]), mn, mx);
