@@ -631,8 +631,9 @@ def main():
                          "that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the codec would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: the codec has no multi-GPU path (a field's "
+              "chunks would be split across GPUs, exchanging only their byte sizes).",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
