@@ -59,7 +59,9 @@ def test_golden_fixtures_legacy_api(szp, golden):
         assert np.array_equal(dec.view(np.uint32), arrays["out_%d" % k]), k
 
 
-FAULT_REASON = ("faulted the GPU twice in this build session: the simple-tile prefetch read x[Et + (uint32_t)(0 - 1)]; fixed (64-bit element index) and checked on the CPU by tests/cpu/geom_check.cc; kept off the GPU until the next session")
+FAULT_REASON = ("faulted the GPU twice: the simple-tile prefetch read x[Et + (uint32_t)(0 - 1)]; "
+                "fixed (64-bit element index) and checked on the CPU by tests/cpu/geom_check.cc; "
+                "not run on the GPU again after the second fault")
 
 CASES = [
     # kind, n, B, T, mode, abs_eb, rel
