@@ -2683,8 +2683,10 @@ __device__ __forceinline__ uint64_t take_ticket(uint64_t *queue, uint32_t ticket
     return *s_t;
 }
 
+// Five workgroups per CU (at most 96 VGPRs, a few bytes spilled): measured
+// on 256 uneven fields, 1.20 -> 1.39 TB/s against three per CU unbounded.
 template <bool INDEX, int SPB>
-__global__ __launch_bounds__(kWG) void k_batch_compress(const BatchJob *__restrict__ jobs,
+__global__ __launch_bounds__(kWG, 5) void k_batch_compress(const BatchJob *__restrict__ jobs,
                                                         const QEntry *__restrict__ qmap,
                                                         uint64_t total, uint64_t *queue,
                                                         uint32_t ticket, uint32_t *ctl)
