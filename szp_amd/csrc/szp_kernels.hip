@@ -1172,11 +1172,18 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     if (L.s_stop) return;
     if (INDEX) {
         const uint32_t nti = (uint32_t)(tIN - tI0);
-        for (uint32_t k = tid; k < nti; k += kWG) {
-            uint32_t t = t_first;
-            uint64_t j = j_first;
-            advance(g, t, j, k);
-            if (item_indexed(g, j)) idx_pos[(uint64_t)t * g.nb0 + item_block(g, j)] = G0 + s_ioff[k];
+        if (simple) {
+            // Blocks cj .. cj + nti - 1 of one chunk: consecutive index entries.
+            uint64_t *ip = idx_pos + (uint64_t)t_first * g.nb0 + (j_first - 1);
+            for (uint32_t k = tid; k < nti; k += kWG) ip[k] = G0 + s_ioff[k];
+        } else {
+            for (uint32_t k = tid; k < nti; k += kWG) {
+                uint32_t t = t_first;
+                uint64_t j = j_first;
+                advance(g, t, j, k);
+                if (item_indexed(g, j))
+                    idx_pos[(uint64_t)t * g.nb0 + item_block(g, j)] = G0 + s_ioff[k];
+            }
         }
     }
     // Chunk offsets relative to the body (SZp/src/szp_float.cc:280-288).
