@@ -1157,6 +1157,11 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
         }
     }
 
+    // The error word is read before the look-back, so that its latency
+    // overlaps the wait instead of following it (a timeout raised during
+    // this tile's own look-back still stops later tiles, and this tile's
+    // writes stay within the capacity check below).
+    const uint32_t err0 = tid == 0 ? ld_ctl(&ctl[1]) : 0u;
     {
         const uint64_t p = lookback_sum_wg(status, tile, 0, run, ctl, published, s_lb);
         if (tid == 0) s_prefix = p;
@@ -1165,7 +1170,7 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     const uint64_t G0 = g.body + s_prefix;
     if (tIN == g.nitems && tid == 0) *d_size = G0 + run;
     if (tid == 0) {
-        L.s_stop = G0 + run > cap || (ld_ctl(&ctl[1]) & kErrTimeout);
+        L.s_stop = G0 + run > cap || (err0 & kErrTimeout);
         if (G0 + run > cap) raise_error(ctl, kErrCapacity);
     }
     __syncthreads();
