@@ -1021,7 +1021,9 @@ struct CompressLds {
 
 // One compress tile (all threads of the workgroup call it; every exit is
 // uniform, so a persistent workgroup can go on to its next tile).
-template <typename V, bool INDEX, int SPB>
+// EARLY_ERR: read the error word before the look-back (the single-field
+// kernel; the batched kernel, at its register limit, reads it after).
+template <typename V, bool INDEX, int SPB, bool EARLY_ERR = true>
 __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_eb,
                               V eb_host, const Geom &g, uint8_t *__restrict__ out,
                               uint64_t cap, uint64_t *__restrict__ status,
@@ -1161,7 +1163,7 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     // overlaps the wait instead of following it (a timeout raised during
     // this tile's own look-back still stops later tiles, and this tile's
     // writes stay within the capacity check below).
-    const uint32_t err0 = tid == 0 ? ld_ctl(&ctl[1]) : 0u;
+    const uint32_t err0 = (EARLY_ERR && tid == 0) ? ld_ctl(&ctl[1]) : 0u;
     {
         const uint64_t p = lookback_sum_wg(status, tile, 0, run, ctl, published, s_lb);
         if (tid == 0) s_prefix = p;
@@ -1170,7 +1172,7 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     const uint64_t G0 = g.body + s_prefix;
     if (tIN == g.nitems && tid == 0) *d_size = G0 + run;
     if (tid == 0) {
-        L.s_stop = G0 + run > cap || (err0 & kErrTimeout);
+        L.s_stop = G0 + run > cap || ((EARLY_ERR ? err0 : ld_ctl(&ctl[1])) & kErrTimeout);
         if (G0 + run > cap) raise_error(ctl, kErrCapacity);
     }
     __syncthreads();
@@ -2758,7 +2760,7 @@ __global__ __launch_bounds__(kWG, 5) void k_batch_compress(const BatchJob *__res
             const QEntry e = qmap[q];
             const BatchJob &J = jobs[e.job];
             const Geom g = J.g;
-            compress_tile<float, INDEX, SPB>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl,
+            compress_tile<float, INDEX, SPB, false>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl,
                                              J.d_size, J.idx_pos, J.idx_carry, e.tile, L);
             __syncthreads();
         }
