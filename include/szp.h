@@ -77,6 +77,33 @@ float *szp_float_decompress_openmp_threadblock(size_t nbEle, float absErrBound, 
 void szp_float_decompress_openmp_threadblock_arg(float *newData, size_t nbEle, float absErrBound,
                                                  int blockSize, unsigned char *cmpBytes);
 
+/* Replaces szp_float_single_thread_arg, SZp/include/szp_float.h:36-37
+ * (SZp/src/szp_float.cc:482-605): the fastMode-1 stream of a team of one
+ * (T = 1, whatever szp_amd_default_threads() says).  `output` must hold
+ * szp_amd_compress_bound(nbEle, blockSize, 1) bytes. */
+void szp_float_single_thread_arg(unsigned char *output, float *oriData, size_t *outSize,
+                                 float absErrBound, size_t nbEle, int blockSize);
+
+/* Replaces szp_float_single_thread_arg_record, SZp/include/szp_float.h:39-40
+ * (SZp/src/szp_float.cc:607-750): as szp_float_single_thread_arg, returning
+ * the memory-access count the reference's instrumented loop tallies for the
+ * same stream (0 on failure). */
+size_t szp_float_single_thread_arg_record(unsigned char *output, float *oriData, size_t *outSize,
+                                          float absErrBound, size_t nbEle, int blockSize);
+
+/* Replaces szp_float_decompress_single_thread_arg, SZp/include/szpd_float.h:25
+ * (SZp/src/szpd_float.cc:130-214): decodes a T = 1 stream; cmpBytes points
+ * just after the 4-byte eb. */
+void szp_float_decompress_single_thread_arg(float *newData, size_t nbEle, float absErrBound,
+                                            int blockSize, unsigned char *cmpBytes);
+
+/* Replaces szp_float_decompress_single_thread_arg_record, SZp/include/szpd_float.h:27
+ * (SZp/src/szpd_float.cc:216-317): as szp_float_decompress_single_thread_arg,
+ * returning the reference's memory-access count (0 on failure). */
+size_t szp_float_decompress_single_thread_arg_record(float *newData, size_t nbEle,
+                                                     float absErrBound, int blockSize,
+                                                     unsigned char *cmpBytes);
+
 /* Replaces szp_float_openmp_threadblock_randomaccess, SZp/include/szp_float.h:42-44
  * (SZp/src/szp_float.cc:752-912): fastMode 2, each chunk cut into units of
  * blockSize values, a unit = the raw quantized first value and one block of

@@ -597,3 +597,43 @@ uint64_t szpo_compress_bound_mode(int fast, int dtype, uint64_t n, int B, int T)
     }
     return total;
 }
+
+/* The memory-access tally of the reference's instrumented single-thread
+ * forms, replayed event by event over a T = 1 fastMode-1 float stream body
+ * (the bytes after the eb): compress SZp/src/szp_float.cc:607-750,
+ * decompress SZp/src/szpd_float.cc:216-317.  Sizes follow the reference's
+ * types: size_t 8, float 4, int 4, unsigned char 1. */
+uint64_t szpo_record_count(const uint8_t *body, uint64_t n, int B, int decompress)
+{
+    uint64_t c = 8;                         /* offsets[0] */
+    uint64_t chunk = 0;                     /* outSize_perthread */
+    uint64_t pos;
+    memcpy(&pos, body, 8);
+    pos += 8;
+    if (n > 0) {
+        c += decompress ? 4 * 2 + 4 * 2     /* head read + prior; value + output */
+                        : 4 + 4 * 2;        /* input read; head memcpy */
+        pos += 4;
+        chunk += 4;
+    }
+    for (uint64_t i = 1; i < n; i += (uint64_t)B) {
+        const uint64_t L = (i + (uint64_t)B > n) ? n - i : (uint64_t)B;
+        const int b = body[pos];
+        const uint64_t bytes = szpo_block_bytes(b, L);
+        const uint64_t signs = (L - 1) / 8 + 1, packed = bytes - 1 - (b ? signs : 0);
+        c += 1;                             /* the width byte */
+        if (!decompress) {
+            for (uint64_t j = 0; j < L; ++j) c += 4 + 1 + 4;  /* input, temp sign, temp mag */
+            if (b) c += L + signs + 4 * L + packed;
+        } else if (!b) {
+            for (uint64_t j = 0; j < L; ++j) c += 4 * 2;
+        } else {
+            c += signs + L + packed + 4 * L;
+            for (uint64_t j = 0; j < L; ++j) c += 1 + 4 + 4 * 2;
+        }
+        pos += bytes;
+        chunk += bytes;
+    }
+    if (!decompress) c += 8 + 8 + 8 * 3 + 8 * 2 + chunk * 2;
+    return c;
+}

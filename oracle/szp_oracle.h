@@ -98,6 +98,13 @@ int szpo_decompress_mode(int fast, int dtype, const uint8_t *stream, uint64_t le
                          int B, int T, int workers, void *out);
 uint64_t szpo_compress_bound_mode(int fast, int dtype, uint64_t n, int B, int T);
 
+/* Memory-access count returned by szp_float_single_thread_arg_record
+ * (decompress = 0, SZp/src/szp_float.cc:607-750) or
+ * szp_float_decompress_single_thread_arg_record (decompress = 1,
+ * SZp/src/szpd_float.cc:216-317) for the T = 1 stream whose body (the bytes
+ * after the 4-byte eb) is given. */
+uint64_t szpo_record_count(const uint8_t *body, uint64_t n, int B, int decompress);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,10 @@ def _load():
         "szp_float_openmp_threadblock_arg": (None, [vp, vp, ctypes.POINTER(szt), f32, szt, i32]),
         "szp_float_decompress_openmp_threadblock": (vp, [szt, f32, i32, vp]),
         "szp_float_decompress_openmp_threadblock_arg": (None, [vp, szt, f32, i32, vp]),
+        "szp_float_single_thread_arg": (None, [vp, vp, ctypes.POINTER(szt), f32, szt, i32]),
+        "szp_float_single_thread_arg_record": (szt, [vp, vp, ctypes.POINTER(szt), f32, szt, i32]),
+        "szp_float_decompress_single_thread_arg": (None, [vp, szt, f32, i32, vp]),
+        "szp_float_decompress_single_thread_arg_record": (szt, [vp, szt, f32, i32, vp]),
         "szp_amd_compress_bound": (u64, [u64, i32, i32]),
         "szp_amd_compress_workspace_size": (u64, [u64, i32, i32]),
         "szp_amd_decompress_workspace_size": (u64, [u64, u64, i32, i32]),

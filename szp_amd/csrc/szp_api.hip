@@ -97,33 +97,72 @@ uint64_t read_u64(const unsigned char *p)
 // double), from the chunk offsets and a walk over the last chunk's width
 // bytes; 0 when malformed.  fastMode 2 walks the last chunk's units, each a
 // 4-byte head and a block of len - 1 values (SZp/src/szpd_float.cc:470-520).
-uint64_t body_length(const unsigned char *body, uint64_t n, int B, int T, int fast = 1)
+// The reference trusts the header; here every position read is first checked
+// against the longest body a valid stream can have (compress_bound less the
+// eb), so a damaged last-chunk offset is refused instead of read through.
+uint64_t body_length(const unsigned char *body, uint64_t n, int B, int T, int fast = 1,
+                     int dtype = 0)
 {
     Geom g;
     if (!make_geom(n, B, T, &g, 4, (uint32_t)fast)) return 0;
     const uint64_t hdr = 8ull * T;
     if (n == 0) return hdr;
+    const uint64_t lim = compress_bound(n, B, T, dtype, fast) - (dtype == SZ_DOUBLE ? 8 : 4);
+    const uint64_t last = read_u64(body + hdr - 8);
+    if (last > lim - hdr) return 0;
     const uint64_t nl = n - (uint64_t)(T - 1) * g.tbs;
     if (fast == 2) {
-        uint64_t pos = hdr + read_u64(body + hdr - 8);
+        uint64_t pos = hdr + last;
         for (uint64_t k = 0; k < g.nblast; ++k) {
             const uint64_t rem = nl - k * (uint64_t)B;
             const uint64_t L = (rem < (uint64_t)B ? rem : (uint64_t)B) - 1;
             pos += 4;
+            if (pos >= lim) return 0;
             const uint32_t b = body[pos];
             if (L && b > 32) return 0;
             pos += L ? block_bytes(b, (uint32_t)L) : 1;
         }
-        return pos;
+        return pos <= lim ? pos : 0;
     }
-    uint64_t pos = hdr + read_u64(body + hdr - 8) + 4;
+    uint64_t pos = hdr + last + 4;
     for (uint64_t k = 0; k < g.nblast; ++k) {
+        if (pos >= lim) return 0;
         const uint32_t b = body[pos];
         if (b > 32) return 0;
         const uint64_t L = (k + 1 == g.nblast) ? nl - 1 - k * (uint64_t)B : (uint64_t)B;
         pos += block_bytes(b, (uint32_t)L);
     }
-    return pos;
+    return pos <= lim ? pos : 0;
+}
+
+// Memory-access count returned by the reference's instrumented single-thread
+// forms, from the block widths of a T = 1 stream body (after the eb):
+//   compress   SZp/src/szp_float.cc:607-750: 8 for the offset; 4 + 8 for the
+//              head; per block of L values 4L (input) + 5L (temporaries) + 1
+//              (width), and for a width b > 0 also L + 4L (reading the
+//              temporaries) + the block's sign, plane and residual bytes;
+//              then 8 + 8 + 24 + 16 of bookkeeping and twice the chunk bytes;
+//   decompress SZp/src/szpd_float.cc:216-317: 8 for the offset; 8 + 8 for the
+//              head; per block 1 (width), then 8L for b = 0, else its sign,
+//              plane and residual bytes + L + 4L + 13L.
+// The body is a stream this library wrote or accepted (widths <= 32).
+size_t single_thread_record(const unsigned char *body, uint64_t n, int B, bool decompress)
+{
+    uint64_t comp = 8, dec = 8;
+    if (n == 0) return decompress ? dec : comp + 56;
+    comp += 12;
+    dec += 16;
+    uint64_t pos = 8 + read_u64(body) + 4, chunk = 4;
+    for (uint64_t i = 1; i < n; i += (uint64_t)B) {
+        const uint64_t L = std::min<uint64_t>((uint64_t)B, n - i);
+        const uint32_t b = body[pos];
+        const uint64_t bb = block_bytes(b, (uint32_t)L);
+        pos += bb;
+        chunk += bb;
+        comp += 9 * L + 1 + (b ? 5 * L + bb - 1 : 0);
+        dec += 1 + (b ? bb - 1 + 18 * L : 8 * L);
+    }
+    return decompress ? dec : comp + 56 + 2 * chunk;
 }
 
 // Bytes per value of a dataType (SZ_FLOAT 4, SZ_DOUBLE 8).
@@ -175,7 +214,7 @@ int host_decompress(int fast, int dtype, const unsigned char *eb_be, const unsig
                     uint64_t n, int B, int T, void *into, void **result)
 {
     const uint64_t vb = value_bytes(dtype);
-    const uint64_t blen = body_length(body, n, B, T, fast);
+    const uint64_t blen = body_length(body, n, B, T, fast, dtype);
     if (!blen) return SZP_AMD_ERR_STREAM;
     const uint64_t len = vb + blen;
     Session *s = session_for_current_device();
@@ -225,7 +264,7 @@ void double_to_be(double v, unsigned char b[8])
 // into `into` when given (the _arg forms), else returns a malloc'd stream.
 unsigned char *lower_compress(const char *who, int fast, int dtype, const void *data,
                               size_t *outSize, double eb, size_t nbEle, int blockSize,
-                              unsigned char *into)
+                              unsigned char *into, int T = 0)
 {
     if (!outSize || (!data && nbEle)) {
         set_error(who, SZP_AMD_ERR_ARG);
@@ -234,7 +273,7 @@ unsigned char *lower_compress(const char *who, int fast, int dtype, const void *
     unsigned char *stream = NULL;
     size_t size = 0;
     const int rc = host_compress(fast, dtype, data, nbEle, ABS, eb, 0.0, blockSize,
-                                 szp_amd_default_threads(), into, &stream, &size);
+                                 T > 0 ? T : szp_amd_default_threads(), into, &stream, &size);
     if (rc) {
         set_error(who, rc);
         *outSize = 0;
@@ -249,7 +288,7 @@ unsigned char *lower_compress(const char *who, int fast, int dtype, const void *
 // passes as a value; writes into `into` when given (the _arg forms), else
 // returns a malloc'd array.
 void *lower_decompress(const char *who, int fast, int dtype, double eb, size_t nbEle,
-                       int blockSize, unsigned char *cmpBytes, void *into)
+                       int blockSize, unsigned char *cmpBytes, void *into, int T = 0)
 {
     if (!cmpBytes) {
         set_error(who, SZP_AMD_ERR_ARG);
@@ -262,7 +301,7 @@ void *lower_decompress(const char *who, int fast, int dtype, double eb, size_t n
         float_to_be((float)eb, ebb);
     void *out = NULL;
     const int rc = host_decompress(fast, dtype, ebb, cmpBytes, nbEle, blockSize,
-                                   szp_amd_default_threads(), into, &out);
+                                   T > 0 ? T : szp_amd_default_threads(), into, &out);
     if (rc) {
         set_error(who, rc);
         return NULL;
@@ -501,7 +540,7 @@ uint64_t szp_amd_stream_length_typed(int fastMode, int dataType, const unsigned 
 {
     if (!bytes || !typed_ok(fastMode, dataType)) return 0;
     const uint64_t vb = value_bytes(dataType);
-    const uint64_t b = body_length(bytes + vb, nbEle, blockSize, nbThreads, fastMode);
+    const uint64_t b = body_length(bytes + vb, nbEle, blockSize, nbThreads, fastMode, dataType);
     return b ? vb + b : 0;
 }
 
@@ -597,6 +636,55 @@ void szp_float_decompress_openmp_threadblock_arg(float *newData, size_t nbEle, f
     }
     lower_decompress("szp_float_decompress_openmp_threadblock_arg", 1, SZ_FLOAT, absErrBound,
                      nbEle, blockSize, cmpBytes, newData);
+}
+
+void szp_float_single_thread_arg(unsigned char *output, float *oriData, size_t *outSize,
+                                 float absErrBound, size_t nbEle, int blockSize)
+{
+    if (!output) {
+        set_error("szp_float_single_thread_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_compress("szp_float_single_thread_arg", 1, SZ_FLOAT, oriData, outSize,
+                   (double)absErrBound, nbEle, blockSize, output, 1);
+}
+
+size_t szp_float_single_thread_arg_record(unsigned char *output, float *oriData, size_t *outSize,
+                                          float absErrBound, size_t nbEle, int blockSize)
+{
+    if (!output) {
+        set_error("szp_float_single_thread_arg_record", SZP_AMD_ERR_ARG);
+        return 0;
+    }
+    if (!lower_compress("szp_float_single_thread_arg_record", 1, SZ_FLOAT, oriData, outSize,
+                        (double)absErrBound, nbEle, blockSize, output, 1))
+        return 0;
+    return single_thread_record(output + 4, nbEle, blockSize, false);
+}
+
+void szp_float_decompress_single_thread_arg(float *newData, size_t nbEle, float absErrBound,
+                                            int blockSize, unsigned char *cmpBytes)
+{
+    if (!newData && nbEle) {
+        set_error("szp_float_decompress_single_thread_arg", SZP_AMD_ERR_ARG);
+        return;
+    }
+    lower_decompress("szp_float_decompress_single_thread_arg", 1, SZ_FLOAT, absErrBound, nbEle,
+                     blockSize, cmpBytes, newData, 1);
+}
+
+size_t szp_float_decompress_single_thread_arg_record(float *newData, size_t nbEle,
+                                                     float absErrBound, int blockSize,
+                                                     unsigned char *cmpBytes)
+{
+    if (!newData && nbEle) {
+        set_error("szp_float_decompress_single_thread_arg_record", SZP_AMD_ERR_ARG);
+        return 0;
+    }
+    if (!lower_decompress("szp_float_decompress_single_thread_arg_record", 1, SZ_FLOAT,
+                          absErrBound, nbEle, blockSize, cmpBytes, newData, 1))
+        return 0;
+    return single_thread_record(cmpBytes, nbEle, blockSize, true);
 }
 
 unsigned char *szp_float_openmp_threadblock_randomaccess(float *oriData, size_t *outSize,

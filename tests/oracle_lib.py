@@ -59,6 +59,8 @@ def oracle():
         O.szpo_decompress_mode.argtypes = [i32, i32, vp, u64, u64, i32, i32, i32, vp]
         O.szpo_compress_bound_mode.restype = u64
         O.szpo_compress_bound_mode.argtypes = [i32, i32, u64, i32, i32]
+        O.szpo_record_count.restype = u64
+        O.szpo_record_count.argtypes = [vp, u64, i32, i32]
         _O = O
     return _O
 
@@ -133,6 +135,12 @@ def reference():
         R.szp_decompress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
                                      ctypes.c_size_t, ctypes.c_int]
         R.szp_ref_set_threads.argtypes = [ctypes.c_int]
+        szt, vp, f32, i32 = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_float, ctypes.c_int
+        R.szp_float_single_thread_arg_record.restype = szt
+        R.szp_float_single_thread_arg_record.argtypes = [vp, vp, ctypes.POINTER(szt), f32, szt,
+                                                         i32]
+        R.szp_float_decompress_single_thread_arg_record.restype = szt
+        R.szp_float_decompress_single_thread_arg_record.argtypes = [vp, szt, f32, i32, vp]
         _R = R
     return _R
 
@@ -173,3 +181,30 @@ def ref_decompress(stream, n, B, T, dtype=np.float32, fast=1):
     out = np.ctypeslib.as_array((ct * n).from_address(p)).copy()
     _libc.free(p)
     return out
+
+
+def record_count(stream, n, B, decompress):
+    """The oracle's memory-access tally of the reference's instrumented
+    single-thread forms for a T = 1 float stream (the whole stream, eb first)."""
+    s = np.ascontiguousarray(stream[4:], dtype=np.uint8)
+    return int(oracle().szpo_record_count(s.ctypes.data, n, B, 1 if decompress else 0))
+
+
+def ref_single_thread_record(x, eb, B):
+    """(stream, compress count, decompressed values, decompress count) from the
+    reference's szp_float_single_thread_arg_record and
+    szp_float_decompress_single_thread_arg_record (x must compress to at most
+    4 * x.size bytes: the reference's per-thread buffer,
+    SZp/src/szp_float.cc:632-633)."""
+    R = reference()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(4 * x.size + 64, np.uint8)
+    sz = ctypes.c_size_t()
+    c1 = R.szp_float_single_thread_arg_record(out.ctypes.data, x.ctypes.data, ctypes.byref(sz),
+                                              eb, x.size, B)
+    stream = out[:sz.value].copy()
+    dec = np.zeros(x.size, np.float32)
+    body = np.ascontiguousarray(stream[4:])
+    c2 = R.szp_float_decompress_single_thread_arg_record(dec.ctypes.data, x.size, eb, B,
+                                                         body.ctypes.data)
+    return stream, int(c1), dec, int(c2)

@@ -194,3 +194,29 @@ def test_raw_file_helpers(tmp_path):
     assert np.array_equal(got.view(np.uint64), y.byteswap().view(np.uint64))
     assert L.szp_readByteData(str(tmp_path / "missing").encode(), ctypes.byref(n),
                               ctypes.byref(st)) is None and st.value == -2
+
+
+def test_damaged_header_is_refused_before_any_read(golden):
+    """szp_amd_stream_length (and so szp_decompress) refuses a stream whose
+    last chunk offset points past the longest body a valid stream can have,
+    instead of walking the host buffer from there; a valid stream keeps its
+    length.  The stream is embedded in a guard buffer whose tail is never
+    reached by a refused walk."""
+    import szp_amd
+    cases, arrays, _ = golden
+    L = szp_amd.lib
+    checked = 0
+    for c in cases:
+        s = arrays["stream_%d" % c["id"]]
+        n, B, T = c["n"], c["B"], c["T"]
+        if n == 0:
+            continue
+        hdr_last = 4 + 8 * (T - 1)
+        bound = szp_amd.compress_bound(n, B, T)
+        for bad in (bound, bound - 4, 1 << 40, (1 << 64) - 1):
+            d = s.copy()
+            d[hdr_last:hdr_last + 8] = np.frombuffer(np.uint64(bad).tobytes(), np.uint8)
+            assert L.szp_amd_stream_length(d.ctypes.data, n, B, T) == 0, (c["id"], bad)
+        assert L.szp_amd_stream_length(s.ctypes.data, n, B, T) == s.size
+        checked += 1
+    assert checked >= 20

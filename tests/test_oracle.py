@@ -275,3 +275,27 @@ def test_oracle_matches_reference_live():
                     assert np.array_equal(d1.view(np.uint32), d2.view(np.uint32))
                     n_checked += 1
     assert n_checked > 50
+
+
+def test_record_count_matches_reference_live():
+    """The oracle's tally of the reference's instrumented single-thread forms
+    (SZp/src/szp_float.cc:607-750, SZp/src/szpd_float.cc:216-317) equals the
+    count the reference itself returns, and the reference's single-thread
+    stream is the oracle's T = 1 stream."""
+    if oracle_lib.reference() is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    rng = np.random.default_rng(5)
+    checked = 0
+    for B in (1, 8, 9, 32, 128):
+        for n in (0, 1, 2, 9, 129, 5000):
+            x = (np.cumsum(rng.standard_normal(n)) * 0.05).astype(np.float32)
+            x[n // 3: n // 2] = 0.0  # runs of zero-width blocks
+            stream, c1, dec, c2 = oracle_lib.ref_single_thread_record(x, np.float32(0.01), B)
+            mine = oracle_lib.compress(x, 0, 0.01, 0.0, B, 1)
+            assert np.array_equal(stream, mine), (B, n)
+            assert oracle_lib.record_count(stream, n, B, False) == c1, (B, n)
+            assert oracle_lib.record_count(stream, n, B, True) == c2, (B, n)
+            ref = oracle_lib.decompress(mine, n, B, 1)
+            assert np.array_equal(dec.view(np.uint32), ref.view(np.uint32))
+            checked += 1
+    assert checked == 30
