@@ -30,33 +30,12 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBPS = 8000.0
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import workloads  # noqa: E402  (the configurations' synthetic fields)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def make_grf(torch, n, seed):
-    """512^3 k^-3 Gaussian random field in [-1, 1], built on the GPU."""
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    w = torch.randn((n, n, n), generator=g, device="cuda", dtype=torch.float32)
-    F = torch.fft.rfftn(w)
-    del w
-    kx = torch.fft.fftfreq(n, device="cuda").view(n, 1, 1)
-    ky = torch.fft.fftfreq(n, device="cuda").view(1, n, 1)
-    kz = torch.fft.rfftfreq(n, device="cuda").view(1, 1, n // 2 + 1)
-    k = torch.sqrt(kx * kx + ky * ky + kz * kz)
-    k[0, 0, 0] = 1.0
-    F *= k.pow(-1.5)
-    F[0, 0, 0] = 0
-    del k
-    x = torch.fft.irfftn(F, s=(n, n, n))
-    del F
-    mn, mx = x.min(), x.max()
-    x = ((x - mn) / (mx - mn) * 2 - 1).contiguous().view(-1)
-    torch.cuda.synchronize()
-    return x
 
 
 def load_reference(threads):
@@ -80,6 +59,92 @@ def load_reference(threads):
         def decomp(s, n, B):
             return oracle_lib.decompress(s, n, B, threads, threads)
     return kind, comp, decomp
+
+
+def cpu_threads():
+    """OpenMP threads of the CPU baseline: OMP_NUM_THREADS when set (the
+    host's CPU share for this GPU), else the CPUs this process may run on."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    if v.isdigit() and int(v) > 0:
+        return int(v), "OMP_NUM_THREADS"
+    return len(os.sched_getaffinity(0)), "sched_getaffinity"
+
+
+def cpu_timing(comp, decomp, xh, rel, B, warmup=2, reps=10):
+    """2 warm-ups, then the median (and min) of `reps` API-level wall times of
+    compress and of decompress, as the reference's drivers time them."""
+    N = xh.size
+    tc, td = [], []
+    stream = dec = None
+    for k in range(warmup + reps):
+        t0 = time.perf_counter()
+        stream = comp(xh, rel, B)
+        t1 = time.perf_counter()
+        dec = decomp(stream, N, B)
+        t2 = time.perf_counter()
+        if k >= warmup:
+            tc.append(t1 - t0)
+            td.append(t2 - t1)
+    return dict(compress_s=statistics.median(tc), decompress_s=statistics.median(td),
+                compress_min_s=min(tc), decompress_min_s=min(td), stream=stream, dec=dec)
+
+
+def c1_bench(torch, szp, n=1 << 26, rel=1e-3, B=128, T=16, seed=12345, steps=10, cpu=True):
+    """The CPU-path configuration: a 64 Mi-element sum of 8 sinusoids plus
+    N(0, 1e-3) noise, REL 1e-3 (eb = 1e-3 x value range), B = 128.  The
+    reference on the host cores (2 warm-ups, median of 10) and the GPU on the
+    same field, with byte parity at the GPU's team size T."""
+    import numpy as np
+    x = workloads.sinusoids(torch, n, seed)
+    codec = szp.DeviceCodec(block_size=B, threads=T)
+    out, size_t, idx = codec.compress(x, szp.REL, 0.0, rel, index=True)
+    codec.check()
+    C = int(size_t.item())
+    dec = torch.empty_like(x)
+    codec.decompress(out, C, n, out=dec, index=idx)
+    codec.check()
+    comp_ms = _timed(torch, lambda: codec.compress(x, szp.REL, 0.0, rel, out=out, index=idx), steps)
+    dec_ms = _timed(torch, lambda: codec.decompress(out, C, n, out=dec, index=idx), steps)
+    dec_parse_ms = _timed(torch, lambda: codec.decompress(out, C, n, out=dec), steps)
+    codec.check()
+    stream_head = out[:8].cpu().numpy()
+    res = {"workload": "1-D float32, %d elements: 8 sinusoids (f log-uniform 1e-6..1e-3 "
+                       "cycles/element, a ~ U(0.1,1)) + N(0,1e-3), seed %d; REL %g, B=%d"
+                       % (n, seed, rel, B),
+           "n_elements": n, "compressed_bytes": C, "ratio": round(4 * n / C, 4),
+           "gpu": {"T": T, "compress_GBps": round(4 * n / (comp_ms * 1e-3) / 1e9, 1),
+                   "decompress_GBps": round(4 * n / (dec_ms * 1e-3) / 1e9, 1),
+                   "decompress_foreign_stream_GBps": round(4 * n / (dec_parse_ms * 1e-3) / 1e9, 1),
+                   "compress_hbm_frac": round((8 * n + C) / (comp_ms * 1e-3) / 8e12, 4),
+                   "decompress_hbm_frac": round((C + 4 * n) / (dec_ms * 1e-3) / 8e12, 4)},
+           "error_bound": eb_report(torch, x, dec, stream_head)}
+    if cpu:
+        threads, src = cpu_threads()
+        kind, rcomp, rdecomp = load_reference(threads)
+        xh = x.cpu().numpy()
+        t = cpu_timing(rcomp, rdecomp, xh, rel, B)
+        err = np.abs(xh.astype(np.float64) - t["dec"].astype(np.float64))
+        res["cpu"] = {"kind": kind, "cores": threads, "cores_from": src,
+                      "compress_GBps": round(4 * n / t["compress_s"] / 1e9, 3),
+                      "decompress_GBps": round(4 * n / t["decompress_s"] / 1e9, 3),
+                      "compress_s_median_of_10": round(t["compress_s"], 4),
+                      "decompress_s_median_of_10": round(t["decompress_s"], 4),
+                      "ratio": round(4 * n / t["stream"].size, 4),
+                      "max_abs_err": float(err.max())}
+        if threads != T:
+            kind_t, rcomp_t, rdecomp_t = load_reference(T)
+            ref = rcomp_t(xh, rel, B)
+            ref_dec = rdecomp_t(ref, n, B)
+        else:
+            ref, ref_dec = t["stream"], t["dec"]
+        got = out[:C].cpu().numpy()
+        res["parity"] = {"against": kind, "T": T,
+                         "stream_byte_identical": bool(ref.size == C and np.array_equal(ref, got)),
+                         "decompress_bit_identical": bool(np.array_equal(
+                             ref_dec.view(np.uint32), dec.cpu().numpy().view(np.uint32)))}
+    del x, out, dec, idx, codec
+    torch.cuda.empty_cache()
+    return res
 
 
 def _timed(torch, fn, steps):
@@ -153,48 +218,6 @@ def eb_report(torch, x, dec, stream_head):
                 strict_over_eb=int((err > eb).sum()), within_rounding_bound=bool((err <= tol).all()))
 
 
-def make_field_1d(torch, kind, n, gen):
-    """One field of the uneven-size mix: 'smooth' (1-D k^-3 spectrum, [-1, 1]),
-    'noise' (U(-1, 1)), or 'zeroblocks' (smooth with 20% of the 128-element
-    index-aligned blocks set to 0)."""
-    if kind == "noise":
-        return torch.rand(n, generator=gen, device="cuda") * 2 - 1
-    m = 1 << max(int(n - 1).bit_length(), 4)  # power-of-two FFT length (few distinct plans)
-    w = torch.randn(m, generator=gen, device="cuda")
-    F = torch.fft.rfft(w)
-    k = torch.fft.rfftfreq(m, device="cuda")
-    k[0] = 1.0
-    F *= k.pow(-1.5)
-    F[0] = 0
-    x = torch.fft.irfft(F, n=m)[:n]
-    mn, mx = x.min(), x.max()
-    x = ((x - mn) / (mx - mn) * 2 - 1).contiguous()
-    if kind == "zeroblocks":
-        nblk = (n + 127) // 128
-        z = torch.rand(nblk, generator=gen, device="cuda") < 0.2
-        m = z.repeat_interleave(128)[:n]
-        x = torch.where(m, torch.zeros_like(x), x)
-    return x.contiguous()
-
-
-def make_mix(torch, count, lo_bytes, hi_bytes, seed):
-    """`count` fields with sizes log-uniform in [lo, hi] bytes, kinds cycling
-    smooth / noise / zeroblocks (seeded)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    sizes = np.exp(rng.uniform(np.log(lo_bytes), np.log(hi_bytes), count)).astype(np.int64) // 4
-    kinds = ["smooth", "noise", "zeroblocks"]
-    xs, ks = [], []
-    for i, n in enumerate(sizes):
-        kind = kinds[rng.integers(0, 3)]
-        xs.append(make_field_1d(torch, kind, int(max(n, 1)), gen))
-        ks.append(kind)
-    torch.cuda.synchronize()
-    return xs, ks
-
-
 def _pct(v, q):
     import numpy as np
     return float(np.percentile(np.asarray(v), q))
@@ -204,7 +227,7 @@ def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
     """Uneven-size mix: per-field plain calls against one batched launch
     (autotuned, replayed from a graph), compress and decompress."""
     from szp_amd import batch as sb
-    xs, kinds = make_mix(torch, count, 64 << 10, 64 << 20, seed)
+    xs, kinds = workloads.mix(torch, count, 64 << 10, 64 << 20, seed)
     N = sum(x.numel() for x in xs)
     log("batched leg: %d fields, %.2f GB generated" % (len(xs), 4 * N / 1e9))
     mode = (szp.REL, 0.0, rel)
@@ -335,7 +358,7 @@ def serving_bench(torch, szp, rate=2000.0, count=400, pool_size=48, B=128, T=16,
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     nbytes = np.exp(rng.uniform(np.log(16 << 10), np.log(256 << 20), pool_size)).astype(np.int64)
-    pool = [make_field_1d(torch, ["smooth", "noise"][rng.integers(0, 2)], int(b // 4), gen)
+    pool = [workloads.field_1d(torch, ["smooth", "noise"][rng.integers(0, 2)], int(b // 4), gen)
             for b in nbytes]
     ebs = [1e-2, 1e-3, 1e-4]
     pre = []
@@ -507,25 +530,7 @@ def large_field_bench(torch, szp, B=64, T=16, rel=1e-4, seed=5, steps=3, cpu_che
     Gaussian noise, REL 1e-4, B=64, compressed and decompressed in one pass
     each (the prefix stage over ~67 M blocks)."""
     import numpy as np
-    nx, ny, nz = 2048, 2048, 1024
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    w = torch.randn((nx, ny, nz), generator=g, device="cuda", dtype=torch.float32)
-    F = torch.fft.rfftn(w)
-    del w
-    kx = torch.fft.fftfreq(nx, device="cuda").view(nx, 1, 1)
-    ky = torch.fft.fftfreq(ny, device="cuda").view(1, ny, 1)
-    kz = torch.fft.rfftfreq(nz, device="cuda").view(1, 1, nz // 2 + 1)
-    F *= (kx * kx + ky * ky + kz * kz).clamp_min(1e-12).pow(-0.75)
-    F[0, 0, 0] = 0
-    x = torch.fft.irfftn(F, s=(nx, ny, nz)).view(-1)
-    del F
-    torch.cuda.empty_cache()
-    x = (x - x.mean()) / x.std()
-    x += 0.01 * torch.randn(x.numel(), generator=g, device="cuda")
-    mn, mx = x.min(), x.max()
-    x = ((x - mn) / (mx - mn) * 2 - 1).contiguous()
-    torch.cuda.synchronize()
+    x = workloads.large_field(torch, seed)
     N = x.numel()
     log("large field: %d elements generated" % N)
     codec = szp.DeviceCodec(block_size=B, threads=T)
@@ -621,6 +626,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the serving leg (JSON line)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving arrivals per second")
     ap.add_argument("--no-large", action="store_true", help="skip the 2^32-element field leg")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 64 Mi sinusoid field leg (reference CPU path + GPU)")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the float64 and fastMode 2 (random access) legs")
     ap.add_argument("--large-only", action="store_true",
@@ -658,7 +665,7 @@ def main():
 
     n3, B, T, rel = args.n, args.block, args.threads, args.rel
     log("generating %d^3 k^-3 field" % n3)
-    x = make_grf(torch, n3, args.seed)
+    x = workloads.grf(torch, n3, args.seed)
     N = x.numel()
     codec = szp.DeviceCodec(block_size=B, threads=T)
     log("main: REL %g B=%d T=%d" % (rel, B, T))
@@ -693,31 +700,33 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu:
-        kind, rcomp, rdecomp = load_reference(T)
         xh = x.cpu().numpy()
-        log("cpu %s: compress + decompress on %d threads" % (kind, T))
-        ts_c, ts_d = [], []
-        ref_stream = ref_dec = None
-        for _ in range(3):
-            t0 = time.perf_counter()
-            ref_stream = rcomp(xh, rel, B)
-            t1 = time.perf_counter()
-            ref_dec = rdecomp(ref_stream, N, B)
-            t2 = time.perf_counter()
-            ts_c.append(t1 - t0)
-            ts_d.append(t2 - t1)
-        tc, td = statistics.median(ts_c), statistics.median(ts_d)
-        cpu = dict(value=round(8 * N / (tc + td) / 1e9, 3), unit="GB/s", cores=T, kind=kind,
-                   sample="the full 512^3 field, REL %g, B=%d: median of 3 (compress %.3f s, "
-                          "decompress %.3f s)" % (rel, B, tc, td),
+        threads, src = cpu_threads()
+        kind, rcomp_h, rdecomp_h = load_reference(threads)
+        log("cpu %s: compress + decompress on %d threads (2 warm-ups, median of 10)"
+            % (kind, threads))
+        t = cpu_timing(rcomp_h, rdecomp_h, xh, rel, B)
+        tc, td = t["compress_s"], t["decompress_s"]
+        cpu = dict(value=round(8 * N / (tc + td) / 1e9, 3), unit="GB/s", cores=threads, kind=kind,
+                   cores_from=src,
+                   sample="the full %d^3 field, REL %g, B=%d, T=%d: 2 warm-ups, then the median "
+                          "of 10 API-level calls (compress %.3f s, decompress %.3f s)"
+                          % (n3, rel, B, threads, tc, td),
                    compress_GBps=round(4 * N / tc / 1e9, 3),
                    decompress_GBps=round(4 * N / td / 1e9, 3))
+        # Parity at the GPU stream's team size T.
+        kind, rcomp, rdecomp = load_reference(T)
+        if threads == T:
+            ref_stream, ref_dec = t["stream"], t["dec"]
+        else:
+            ref_stream = rcomp(xh, rel, B)
+            ref_dec = rdecomp(ref_stream, N, B)
         gpu_dec = r["dec"].cpu().numpy()
         parity = dict(stream_byte_identical=bool(ref_stream.size == C and
                                                  np.array_equal(ref_stream, stream_host)),
                       decompress_bit_identical=bool(np.array_equal(ref_dec.view(np.uint32),
                                                                    gpu_dec.view(np.uint32))),
-                      against=kind)
+                      against=kind, T=T)
 
     variants = []
     if not args.no_variants:
@@ -737,6 +746,11 @@ def main():
                 v["stream_byte_identical"] = bool(rs.size == vs.size and np.array_equal(rs, vs))
             variants.append(v)
             del vc, vr
+
+    c1 = None
+    if not args.no_c1:
+        log("C1: 64 Mi sinusoid field (the CPU-path configuration)")
+        c1 = c1_bench(torch, szp, T=T, cpu=not args.no_cpu)
 
     other = None
     if not args.no_other:
@@ -793,6 +807,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
+        "c1": c1,
         "other_paths": other,
         "batched": batched,
         "serving": served,

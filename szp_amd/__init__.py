@@ -37,7 +37,10 @@ STATUS = {
     7: "allocation failed",
 }
 
-library_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libszp_amd.so")
+# SZP_AMD_LIBRARY names another build of the same library (a side-by-side
+# comparison of two builds in one process tree); by default the in-tree one.
+library_path = os.environ.get("SZP_AMD_LIBRARY") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libszp_amd.so")
 
 
 class SzpError(RuntimeError):

@@ -1693,6 +1693,68 @@ __device__ __forceinline__ void store_octet_out(V *p, const V y[8])
     }
 }
 
+// Stores the eight float values y of elements e .. e + 7 of `out` (p = out + e)
+// with 16-byte-aligned vector stores only.  a = e mod 4 is the same in every
+// lane of a wave (octets start 8 elements apart), so lane l writes the aligned
+// vectors [e - a, e - a + 4) -- the previous lane's last a values (DPP
+// wave_shr:1) and its own first 4 - a -- and [e - a + 4, e - a + 8).  Lane 0
+// writes its own first 4 - a values alone and lane 63 its last a values alone,
+// with ordinary stores: the vector they share with the neighbouring wave is
+// completed by the two.  All 64 lanes must call it (the DPP reads every lane).
+// NT: the aligned vectors are non-temporal (misaligned non-temporal vectors
+// write partial lines: measured 1.28x the output bytes on a 512^3 decode).
+template <bool NT, int A>
+__device__ __forceinline__ void store_octet_aligned_a(float *p, const float y[8])
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int lane = __lane_id();
+    float prev[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < A; ++k)
+        prev[k] = __int_as_float(__builtin_amdgcn_update_dpp(
+            0, __float_as_int(y[8 - A + k]), 0x138, 0xF, 0xF, false));  // wave_shr:1
+    f4 v1, v2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v1[k] = k < A ? prev[k] : y[k - A];
+        v2[k] = y[4 - A + k];
+    }
+    f4 *q1 = reinterpret_cast<f4 *>(p - A), *q2 = reinterpret_cast<f4 *>(p + 4 - A);
+    if (A == 0 || lane != 0) {
+        if (NT) __builtin_nontemporal_store(v1, q1); else *q1 = v1;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4 - A; ++k) p[k] = y[k];
+    }
+    if (NT) __builtin_nontemporal_store(v2, q2); else *q2 = v2;
+    if (A != 0 && lane == 63) {
+#pragma unroll
+        for (int k = 8 - A; k < 8; ++k) p[k] = y[k];
+    }
+}
+
+template <bool NT>
+__device__ __forceinline__ void store_octet_aligned(float *p, const float y[8])
+{
+    switch ((reinterpret_cast<uintptr_t>(p) >> 2) & 3) {
+    case 0: store_octet_aligned_a<NT, 0>(p, y); break;
+    case 1: store_octet_aligned_a<NT, 1>(p, y); break;
+    case 2: store_octet_aligned_a<NT, 2>(p, y); break;
+    default: store_octet_aligned_a<NT, 3>(p, y); break;
+    }
+}
+
+// The octet stores of the simple-tile decodes: floats through the aligned
+// form above; doubles (already 8-byte aligned) with ordinary stores.
+template <bool NT, typename V>
+__device__ __forceinline__ void store_octet_tile(V *p, const V y[8])
+{
+    if constexpr (sizeof(V) == 4)
+        store_octet_aligned<NT>(p, y);
+    else
+        store_octet_out<false>(p, y);
+}
+
 // Unpacks one slot (SZp/src/szpd_float.cc:81-110, SZp/src/szp_TypeManager.cc:436-944).
 // The slot's item starts at buf[lp]; lend bounds the bytes present (reads may
 // run up to 32 bytes past it, inside the buffer's slack).  dl[i] = inclusive
@@ -1974,7 +2036,7 @@ __device__ __forceinline__ bool decode_simple_fast(
             V y[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
-            store_octet_out<sizeof(V) == 4>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
+            store_octet_tile<true>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
         }
     }
     return true;
@@ -2956,6 +3018,60 @@ uint32_t *workspace_error_word(void *d_work)
 // Status codes shared with the public header (szp_amd_status).
 enum { kOk = 0, kArg = 1, kCap = 2, kHip = 3 };
 
+// Look-back kernels (k_compress, and k_parse + k_decode without the side
+// index) take their tiles in workgroup order, and a tile waits for the tiles
+// before it: a launch makes progress because its lowest unfinished tile is
+// always resident.  Two such launches on two streams could fill the device
+// with each other's waiting workgroups (every wait is bounded, so that would
+// end in SZP_AMD_ERR_TIMEOUT, not a hang).  The library therefore orders its
+// look-back launches across streams, per device: a launch on stream s first
+// waits for the previous look-back launch when that was on another stream.
+// Launches on one stream are ordered anyway; a stream being captured into a
+// graph is left alone (the graph replays on one stream).
+class LookbackOrder {
+  public:
+    explicit LookbackOrder(hipStream_t s) : s_(s)
+    {
+        int dev = 0;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevs ||
+            hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+            return;
+        d_ = &slots()[dev];
+        d_->mu.lock();
+        if (!d_->ev && hipEventCreateWithFlags(&d_->ev, hipEventDisableTiming) != hipSuccess)
+            d_->ev = nullptr;
+        if (d_->ev && d_->valid && d_->last != s) (void)hipStreamWaitEvent(s, d_->ev, 0);
+    }
+    ~LookbackOrder()
+    {
+        if (!d_) return;
+        if (d_->ev && hipEventRecord(d_->ev, s_) == hipSuccess) {
+            d_->last = s_;
+            d_->valid = true;
+        }
+        d_->mu.unlock();
+    }
+    LookbackOrder(const LookbackOrder &) = delete;
+    LookbackOrder &operator=(const LookbackOrder &) = delete;
+
+  private:
+    static constexpr int kDevs = 64;
+    struct Slot {
+        std::mutex mu;
+        hipEvent_t ev = nullptr;
+        hipStream_t last = nullptr;
+        bool valid = false;
+    };
+    static Slot *slots()
+    {
+        static Slot s[kDevs];
+        return s;
+    }
+    hipStream_t s_;
+    Slot *d_ = nullptr;
+};
+
 // k_compress with the fast simple-tile path compiled for SPB = B / 8 when
 // that is a power of two up to 32 (B = 8 ... 256), else without it.
 template <typename V, bool INDEX>
@@ -3041,6 +3157,7 @@ static int launch_compress_t(int mode, const V *d_x, uint64_t n, int err_mode, d
 
     const uint64_t nst = g.ntiles ? g.ntiles : 1;
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((nst + kWG - 1) / kWG, 4096);
+    LookbackOrder order(stream);
     SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
     const V *d_eb = nullptr;
     // ABS: cast to the value type (float: SZp/src/szp.cc:32; double: :62).
@@ -3118,6 +3235,7 @@ static int launch_decompress_t(int mode, const uint8_t *d_stream, uint64_t len, 
     const uint32_t S = parse_tile_bytes(B, mode);
     const uint32_t W = parse_max_step(B, mode);
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((w.nclr + kWG - 1) / kWG, 4096);
+    LookbackOrder order(stream);
     SZP_LAUNCH("k_dsetup", k_dsetup, dim3(cgrid ? cgrid : 1), dim3(kWG), 0, stream, d_stream, len, g,
                        S, w.ctab, w.exit_st, w.nclr, w.ctl);
     if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;

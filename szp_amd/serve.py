@@ -8,26 +8,38 @@ batch and/or one decompress batch, szp_amd.batch).  Requests that arrive
 while batches run are admitted as memory frees and go into the next batch,
 so work from several batches overlaps on the device.  Each request's result
 is the same bytes as a single-field call on the same input.
+
+A request is charged, at admission, for everything the server allocates for
+it: its output (stream buffer or values), its side index, and its share of a
+batch workspace (the workspace of a batch of that request alone, which bounds
+its share of any batch).  A request larger than the whole budget is refused.
+A request can be cancelled until its batch is launched (Server.cancel or
+Request.cancel_at); its budget is released at once.
 """
 import collections
 import time
 
 from . import REL, compress_bound, lib
-from .batch import CompressBatch, DecompressBatch
+from .batch import CompressBatch, DecompressBatch, Field
 
 
 class Request:
     __slots__ = ("rid", "kind", "x", "mode", "stream", "nbytes", "n", "index", "arrival",
-                 "admitted", "launched", "done", "result", "result_size", "result_index", "need")
+                 "admitted", "launched", "done", "result", "result_size", "result_index", "need",
+                 "cancel_at", "state")
 
     def __init__(self, rid, kind, arrival, x=None, mode=(REL, 0.0, 1e-3), stream=None, nbytes=0,
-                 n=0, index=None):
+                 n=0, index=None, cancel_at=None):
         self.rid, self.kind, self.arrival = rid, kind, arrival
         self.x, self.mode = x, mode
         self.stream, self.nbytes, self.n, self.index = stream, nbytes, n, index
         self.admitted = self.launched = self.done = None
         self.result = self.result_size = self.result_index = None
         self.need = 0
+        # Seconds from the start at which the client cancels (None: never).
+        self.cancel_at = cancel_at
+        # "pending", "admitted", "launched", "done", "cancelled" or "refused".
+        self.state = "pending"
 
 
 class Server:
@@ -50,12 +62,39 @@ class Server:
         self.used = 0
         self.peak = 0
         self.batches = 0
+        self.cancelled = 0
+        self.refused = 0
+        self._cancel = set()
+
+    def cancel(self, rid):
+        """Asks for request `rid` to be cancelled (effective if its batch has
+        not been launched yet; may be called from another thread)."""
+        self._cancel.add(rid)
+
+    def _workspace(self, r):
+        """Bytes of a batch workspace for request r alone."""
+        if r.kind == "compress":
+            em, ab, rel = r.mode
+            f = Field(None, None, r.x.numel(), 0, 0, None, int(em), float(ab), float(rel))
+        else:
+            f = Field(None, None, int(r.n), int(r.nbytes), 0, None, 0, 0.0, 0.0)
+        arr = (Field * 1)(f)
+        return int(lib.szp_amd_batch_workspace_size(arr, 1, self.B, self.T,
+                                                    0 if r.kind == "compress" else 1))
 
     def _need(self, r):
+        """Bytes the server allocates for r: output, side index and its
+        share of a batch workspace (each rounded to the 256-byte alignment
+        the batch gives it)."""
+        a = lambda v: (int(v) + 255) // 256 * 256  # noqa: E731
         if r.kind == "compress":
             n = r.x.numel()
-            return compress_bound(n, self.B, self.T) + int(lib.szp_amd_index_size(n, self.B, self.T)) + 4096
-        return 4 * r.n + 4096
+            return (a(compress_bound(n, self.B, self.T)) +
+                    a(lib.szp_amd_index_size(n, self.B, self.T)) + self._workspace(r))
+        return a(4 * r.n) + self._workspace(r)
+
+    def _cancelled(self, r, now):
+        return r.rid in self._cancel or (r.cancel_at is not None and now >= r.cancel_at)
 
     def _launch(self, reqs, stream):
         """One batch for `reqs` on `stream`; returns the objects to keep alive."""
@@ -93,6 +132,7 @@ class Server:
                     r.result = b.outs[k]
             for r in reqs:
                 r.done = t
+                r.state = "done"
                 self.used -= r.need
 
     def run(self, requests, keep_results=True, poll_sleep=20e-6):
@@ -106,6 +146,19 @@ class Server:
         served = []
         while pending or admitted or inflight:
             now = time.perf_counter() - t0
+            # Cancellations of requests not launched yet (their budget is
+            # released at once).
+            if self._cancel or any(r.cancel_at is not None for r in admitted):
+                keep_adm = []
+                for r in admitted:
+                    if self._cancelled(r, now):
+                        self.used -= r.need
+                        r.state, r.done = "cancelled", now
+                        self.cancelled += 1
+                        served.append(r)
+                    else:
+                        keep_adm.append(r)
+                admitted = keep_adm
             # Retire finished batches (frees their memory).
             for si in list(inflight):
                 ev, keep = inflight[si]
@@ -120,13 +173,26 @@ class Server:
             # Admit arrived requests in order while the budget allows.
             while pending and pending[0].arrival <= now:
                 r = pending[0]
+                if self._cancelled(r, now):
+                    pending.popleft()
+                    r.state, r.done = "cancelled", now
+                    self.cancelled += 1
+                    served.append(r)
+                    continue
                 r.need = self._need(r)
-                if self.used + r.need > self.budget and (admitted or inflight):
+                if r.need > self.budget:
+                    pending.popleft()
+                    r.state, r.done = "refused", now
+                    self.refused += 1
+                    served.append(r)
+                    continue
+                if self.used + r.need > self.budget:
                     break
                 pending.popleft()
                 self.used += r.need
                 self.peak = max(self.peak, self.used)
                 r.admitted = now
+                r.state = "admitted"
                 admitted.append(r)
             # Launch everything admitted on a free stream.
             free = [i for i in range(len(self.streams)) if i not in inflight]
@@ -135,11 +201,12 @@ class Server:
                 tl = time.perf_counter() - t0
                 for r in admitted:
                     r.launched = tl
+                    r.state = "launched"
                 inflight[si] = self._launch(admitted, self.streams[si])
                 self.batches += 1
                 admitted = []
                 continue
-            if not inflight and not admitted and pending:
+            if not inflight and not admitted and pending and pending[0].arrival > now:
                 wait = pending[0].arrival - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(min(wait, 1e-3))
@@ -155,8 +222,13 @@ def percentile(v, q):
 
 
 def report(served, wall):
-    """p50/p99 latency (ms) per kind and aggregate input GB/s."""
+    """p50/p99 latency (ms) per kind and aggregate input GB/s, over the
+    requests served (cancelled and refused ones are counted apart)."""
     out = {}
+    done = [r for r in served if r.state == "done"]
+    out["cancelled"] = sum(1 for r in served if r.state == "cancelled")
+    out["refused"] = sum(1 for r in served if r.state == "refused")
+    served = done
     for kind in ("compress", "decompress"):
         lat = [(r.done - r.arrival) * 1e3 for r in served if r.kind == kind]
         qd = [(r.launched - r.arrival) * 1e3 for r in served if r.kind == kind]
