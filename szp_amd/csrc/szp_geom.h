@@ -18,7 +18,7 @@ constexpr int kWG = 256;                 // threads per workgroup (4 wave64)
 constexpr int kMaxBlock = 2048;          // largest SZp block size handled on the GPU
 constexpr int kOutBufBytes = 8704 + 64;  // LDS bytes for one round's encoded blocks
 constexpr int kMaxRounds = 4;            // rounds of K items per tile
-constexpr int kOutCap = 20480;           // LDS bytes buffering a compress tile before placement
+constexpr int kOutCap = 17408;           // LDS bytes buffering a compress tile before placement
 constexpr int kDecTileBytes = 20480;     // LDS bytes holding a decode tile's stream bytes
 
 // Geometry of the item sequence of one field.  The stream body is the
@@ -190,6 +190,20 @@ SZP_HD bool unit_tile(const Geom &g, uint32_t ct, uint64_t cj, uint32_t nti)
 SZP_HD uint32_t simple_octet_off(const Geom &g, uint32_t r, uint32_t d, uint32_t o)
 {
     return (r * g.K + d) * g.B + 8 * o;
+}
+
+// The simple compress path's wave mapping (spb a power of two up to 32): lane
+// l of wave w holds, in step s, octet l % spb of the tile's block
+// simple_wave_block(...), i.e. the tile's elements simple_wave_off(...) +
+// [0, 8): wave w covers the tile's w-th quarter (K blocks, 2048 elements) in
+// four steps of 512.
+SZP_HD uint32_t simple_wave_block(const Geom &g, uint32_t w, uint32_t s, uint32_t l)
+{
+    return w * g.K + s * (64 / g.spb) + l / g.spb;
+}
+SZP_HD uint32_t simple_wave_off(uint32_t w, uint32_t s, uint32_t l)
+{
+    return 2048 * w + 512 * s + 8 * l;
 }
 
 // One thread's octet of a round: up to 8 consecutive elements of one item.

@@ -784,104 +784,195 @@ __device__ __forceinline__ double lane_before(double v, double first)
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// Fast path of k_compress for a simple tile (only full blocks of one chunk)
-// when SPB = B / 8 is a power of two up to 32.  Then K * SPB = 256: round r
-// is the 2048 elements [r * 2048, (r + 1) * 2048) of the tile and thread tid
-// owns elements r * 2048 + 8 * tid + [0, 8).  The whole tile is loaded up
-// front; each block's maximum is reduced across its SPB lanes with DPP; one
-// wave scan per pair of rounds and one barrier give every block its byte
-// offset in the tile; the tile's byte total is published for the look-back
-// before any byte is encoded.  Rounds that fit in ob are encoded there
-// (nbuf of them); the caller writes them out and re-encodes the rest.
-template <typename V, int SPB, bool INDEX>
-__device__ __forceinline__ void compress_simple_fast(
-    const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob,
-    uint32_t *s_wtot, uint32_t *s_rbase, uint32_t *s_ioff, uint64_t *status, uint64_t tile,
-    uint64_t &run, uint32_t &nbuf, uint32_t *idx_carry)
+// Copies tot bytes from src (LDS) to out[G, G + tot) with the 64 lanes of one
+// wave: 16-byte stores for the aligned middle, byte stores for the ragged ends.
+__device__ __forceinline__ void copy_out_wave(uint8_t *__restrict__ out, uint64_t G, uint32_t tot,
+                                              const uint8_t *src, uint32_t lane)
 {
-    constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
+    const uint64_t G1 = G + tot;
+    const uint64_t a0 = (G + 15) & ~15ull, a1 = G1 & ~15ull;
+    if (a0 >= a1) {
+        for (uint32_t i = lane; i < tot; i += 64) out[G + i] = src[i];
+        return;
+    }
+    const uint32_t head = (uint32_t)(a0 - G), tail = (uint32_t)(G1 - a1);
+    if (lane < head) out[G + lane] = src[lane];
+    if (lane < tail) out[a1 + lane] = src[(uint32_t)(a1 - G) + lane];
+    const uint32_t nv = (uint32_t)((a1 - a0) >> 4);
+    for (uint32_t v = lane; v < nv; v += 64) {
+        uint4 val;
+        __builtin_memcpy(&val, src + head + 16 * v, 16);  // any LDS byte offset
+        *reinterpret_cast<uint4 *>(out + a0 + 16ull * v) = val;
+    }
+}
+
+// LDS bytes of one wave's region in the simple path (a quarter of ob).
+constexpr uint32_t kWaveCap = (uint32_t)kOutCap / 4;
+
+// Lorenzo deltas of a quantized octet q against prev (the q of the element
+// before it): magnitudes, MSB-first sign byte, maximum (SZp/src/szp_float.cc:220-251).
+// mag = max(d, -d) is exact for every int32 d, INT32_MIN included (it maps to
+// 0x80000000 as the reference's unsigned negation does).
+__device__ __forceinline__ uint32_t octet_mags(const int32_t q[8], int32_t prev, uint32_t mg[8],
+                                               uint32_t &sgn)
+{
+    uint32_t omax = 0, sg = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int32_t dd = (int32_t)((uint32_t)q[i] - (uint32_t)prev);
+        prev = q[i];
+        const uint32_t mag = (uint32_t)max(dd, (int32_t)(0u - (uint32_t)dd));
+        mg[i] = mag;
+        sg = (sg << 1) | ((uint32_t)dd >> 31);
+        omax = max(omax, mag);
+    }
+    sgn = sg;
+    return omax;
+}
+
+// The q of the element before the lane's octet: the previous lane's last q
+// (DPP wave_shr:1); lane 0 keeps qp, the q of its own preceding element.
+__device__ __forceinline__ int32_t q_before(const int32_t q[8], int32_t qp)
+{
+    return __builtin_amdgcn_update_dpp(qp, q[7], 0x138, 0xF, 0xF, false);
+}
+
+// Fast path of k_compress for a simple tile (only full blocks of one chunk)
+// when SPB = B / 8 is a power of two up to 32.  Wave w encodes the tile's
+// w-th quarter, elements [2048 w, 2048 w + 2048) = blocks [w K, (w + 1) K)
+// (K = 256 / SPB), in four steps of 512 elements: lane l holds elements
+// 2048 w + 512 s + 8 l + [0, 8) of step s.  All element loads are issued
+// before the first is used (idle lanes, past the tile's blocks, read the
+// tile's first octet; their values are never used; Et >= 1, since the tile's
+// first item is a block).
+//
+// Pass 1 quantizes every step (the q replace the values in registers), takes
+// each block's maximum over its SPB lanes with DPP, and gives each block its
+// byte offset in the quarter with one wave scan per step; one barrier then
+// gives the tile's total, which is published for the look-back before any
+// byte is encoded, and each quarter's offset in the tile.  Pass 2 takes the
+// deltas again from the q and encodes each wave's quarter into its own LDS
+// region (kWaveCap bytes); the quarters are consecutive in the stream.
+//
+// The wave's results: qbase (its quarter's offset in the tile), sbase[s]
+// (step s's offset in the quarter; sbase[4] = the quarter's bytes), nfit and
+// fitbytes (the steps encoded in the region and their bytes: a step that
+// does not fit, and every later one, is encoded again after the look-back,
+// one step at a time) and boff[s], the offset in the quarter of the lane's
+// block in step s.
+template <typename V, int SPB, bool INDEX>
+__device__ __forceinline__ void compress_simple_waves(
+    const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob, uint32_t *s_wtot,
+    uint64_t *status, uint64_t tile, uint32_t *idx_carry, uint32_t &run, uint32_t &qbase,
+    uint32_t (&sbase)[5], uint32_t &nfit, uint32_t &fitbytes, uint32_t (&boff)[4])
+{
+    constexpr uint32_t K = 256 / SPB, KS = 64 / SPB, B = 8 * SPB;
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
-    const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
-
-    // Every slot loads unconditionally, so that all rounds' loads go out back
-    // to back (a conditional load whose result merges with a default value
-    // makes the compiler wait for it before the next round's loads).  Idle
-    // slots (past the tile's items) read the tile's first octet instead and
-    // their values are never used: their blocks are not live, and a live
-    // lane's predecessor lane is live.  Et >= 1 (the tile's first item is a
-    // block), so e >= 1.
-    V va[kMaxRounds][8];
-    V p0[kMaxRounds];
+    const uint32_t o = (uint32_t)lane % SPB;
+    // The lane's block in step s is b0 + s KS (simple_wave_block), its
+    // elements Et + simple_wave_off(wv, s, lane) + [0, 8).
+    const uint32_t b0 = (uint32_t)wv * K + (uint32_t)lane / SPB;
+    int32_t q[4][8], qp[4];
+    {
+        V va[4][8];
+        V p0[4];
 #pragma unroll
-    for (int r = 0; r < kMaxRounds; ++r) {
-        const uint64_t e =
-            Et + ((r * K + d < nti) ? (uint64_t)r * 2048u + 8u * (uint32_t)tid : 0u);
-        __builtin_memcpy(&va[r][0], x + e, 8 * sizeof(V));
-        // Only lane 0 uses it; every lane loads it (from the lines it reads
-        // anyway) so that no branch splits the loads.
-        p0[r] = x[e - 1];
+        for (int s = 0; s < 4; ++s) {
+            const uint64_t e = Et + ((b0 + (uint32_t)s * KS < nti)
+                                         ? simple_wave_off((uint32_t)wv, (uint32_t)s, (uint32_t)lane)
+                                         : 0u);
+            __builtin_memcpy(&va[s][0], x + e, 8 * sizeof(V));
+            p0[s] = x[e - 1];
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            qp[s] = quantize(p0[s], inv);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) q[s][i] = quantize(va[s][i], inv);
+        }
     }
-
-    // Side index: the q of the element before the tile (thread 0 loaded it).
-    if (INDEX && tid == 0) idx_carry[tile] = (uint32_t)quantize(p0[0], inv);
-    uint32_t mg[kMaxRounds][8], sg[kMaxRounds], bw[kMaxRounds], sz[kMaxRounds];
+    // Side index: the q of the element before the tile (lane 0 of wave 0).
+    if (INDEX && tid == 0) idx_carry[tile] = (uint32_t)qp[0];
+    uint32_t bw[4];
+    uint32_t base = 0;
 #pragma unroll
-    for (int r = 0; r < kMaxRounds; ++r) {
-        const bool live = r * K + d < nti;
-        const V pv = lane_before(va[r][7], p0[r]);
-        const uint32_t omax = item_max<SPB>(octet8_deltas(pv, va[r], inv, mg[r], sg[r]));
-        bw[r] = live ? bit_width(omax) : 0u;
-        sz[r] = live ? (bw[r] ? 1u + SPB * (bw[r] + 1u) : 1u) : 0u;
-    }
-    // Exclusive byte offset of each lane's block within its wave's part of
-    // the round: one wave scan per pair of rounds (16-bit halves; a wave's
-    // part of a round is at most 64 / SPB * (1 + 33 * SPB) < 2^16 bytes).
-    uint32_t ex[kMaxRounds];
-#pragma unroll
-    for (int r = 0; r < kMaxRounds; r += 2) {
-        const uint32_t v = o == 0 ? (sz[r] | (sz[r + 1] << 16)) : 0u;
+    for (int s = 0; s < 4; ++s) {
+        const bool live = b0 + (uint32_t)s * KS < nti;
+        uint32_t mg[8], sg;
+        const uint32_t omax = item_max<SPB>(octet_mags(q[s], q_before(q[s], qp[s]), mg, sg));
+        bw[s] = live ? bit_width(omax) : 0u;
+        const uint32_t sz = live ? (bw[s] ? 1u + SPB * (bw[s] + 1u) : 1u) : 0u;  // block_bytes(bw, B)
+        const uint32_t v = o == 0 ? sz : 0u;
         const uint32_t incl = wave_incl_scan_u32(v, lane);
-        ex[r] = (incl & 0xFFFFu) - sz[r];
-        ex[r + 1] = (incl >> 16) - sz[r + 1];
-        const uint32_t wt = __builtin_amdgcn_readlane(incl, 63);
-        if (lane == 0) {
-            s_wtot[r * 4 + wv] = wt & 0xFFFFu;
-            s_wtot[(r + 1) * 4 + wv] = wt >> 16;
-        }
+        // sz is the same in every lane of a block, so incl - sz is the
+        // block's start in each of them.
+        boff[s] = base + incl - sz;
+        sbase[s] = base;
+        base += __builtin_amdgcn_readlane(incl, 63);
     }
+    sbase[4] = base;
+    if (lane == 0) s_wtot[wv] = base;
     __syncthreads();
-    uint32_t off[kMaxRounds], rb[kMaxRounds + 1];
-    uint32_t acc = 0;
+    run = 0;
+    qbase = 0;
 #pragma unroll
-    for (int r = 0; r < kMaxRounds; ++r) {
-        uint32_t before = 0, rt = 0;
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const uint32_t t = s_wtot[r * 4 + w];
-            before += w < wv ? t : 0u;
-            rt += t;
-        }
-        rb[r] = acc;
-        off[r] = acc + before + ex[r];
-        acc += rt;
+    for (int w = 0; w < kWG / 64; ++w) {
+        const uint32_t t = s_wtot[w];
+        qbase += w < wv ? t : 0u;
+        run += t;
     }
-    rb[kMaxRounds] = acc;
-    run = acc;
-    if (tid < 64) lookback_publish(status, tile, 0, acc);
-    nbuf = 0;
+    if (tid < 64) lookback_publish(status, tile, 0, run);
+    uint8_t *reg = ob + (uint32_t)wv * kWaveCap;
+    nfit = 0;
+    fitbytes = 0;
 #pragma unroll
-    for (int r = 0; r < kMaxRounds; ++r)
-        if (r * K < nti && rb[r + 1] <= (uint32_t)kOutCap) nbuf = r + 1;
-    if (tid == 0)
-        for (int r = 0; r <= kMaxRounds; ++r) s_rbase[r] = rb[r];
+    for (int s = 0; s < 4; ++s) {
+        if (sbase[s + 1] > kWaveCap) break;  // uniform in the wave
+        // The deltas are taken again: an empty asm statement that "changes"
+        // the q keeps the compiler from holding pass 1's magnitudes (32
+        // registers) across the barrier instead.
 #pragma unroll
-    for (int r = 0; r < kMaxRounds; ++r) {
-        const bool live = r * K + d < nti;
-        if (INDEX && live && o == 0) s_ioff[r * K + d] = off[r];
-        if ((uint32_t)r < nbuf && live) {
-            if (o == 0) ob[off[r]] = (uint8_t)bw[r];
-            if (bw[r]) encode_octet8(ob, off[r], o, B, bw[r], mg[r], sg[r]);
+        for (int i = 0; i < 8; ++i) asm volatile("" : "+v"(q[s][i]));
+        asm volatile("" : "+v"(qp[s]));
+        const int32_t pv = q_before(q[s], qp[s]);  // DPP with every lane active
+        if (b0 + (uint32_t)s * KS < nti) {
+            uint32_t mg[8], sg;
+            octet_mags(q[s], pv, mg, sg);
+            if (o == 0) reg[boff[s]] = (uint8_t)bw[s];
+            if (bw[s]) encode_octet8(reg, boff[s], o, B, bw[s], mg, sg);
         }
+        nfit = s + 1;
+        fitbytes = sbase[s + 1];
+    }
+}
+
+// Step s of a simple tile's quarter again, after the look-back: loads,
+// deltas and widths as in compress_simple_waves, encoded at the start of the
+// wave's region (the step's bytes always fit: at most KS blocks of
+// block_bytes(32, B) bytes).
+template <typename V, int SPB>
+__device__ __forceinline__ void simple_step_again(const V *__restrict__ x, V inv, uint64_t Et,
+                                                  uint32_t nti, uint32_t s, uint8_t *reg,
+                                                  uint32_t off)
+{
+    constexpr uint32_t K = 256 / SPB, KS = 64 / SPB, B = 8 * SPB;
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t o = (uint32_t)lane % SPB;
+    const uint32_t b = (uint32_t)wv * K + s * KS + (uint32_t)lane / SPB;
+    const bool live = b < nti;
+    const uint64_t e = Et + (live ? simple_wave_off((uint32_t)wv, s, (uint32_t)lane) : 0u);
+    V v[8];
+    __builtin_memcpy(&v[0], x + e, 8 * sizeof(V));
+    const int32_t qp = quantize(x[e - 1], inv);
+    int32_t q[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[i] = quantize(v[i], inv);
+    uint32_t mg[8], sg;
+    const uint32_t omax = item_max<SPB>(octet_mags(q, q_before(q, qp), mg, sg));
+    const uint32_t bw = live ? bit_width(omax) : 0u;
+    if (live) {
+        if (o == 0) reg[off] = (uint8_t)bw;
+        if (bw) encode_octet8(reg, off, o, B, bw, mg, sg);
     }
 }
 
@@ -895,14 +986,18 @@ __device__ __forceinline__ void compress_simple_fast(
 // lane (DPP wave_shl:1; the unit's last lane has 7).  A tile holds at most
 // 4096 values, whose stream bytes always fit in ob, so every round is
 // encoded at once and the tile's total is published before any byte is.
-template <typename V, int SPB, bool INDEX>
+// uoff[r]: in the lane that starts a unit (o == 0), the unit's offset in the
+// tile.
+template <typename V, int SPB>
 __device__ __forceinline__ void compress_units_fast(
     const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob, uint32_t *s_wtot,
-    uint32_t *s_ioff, uint64_t *status, uint64_t tile, uint64_t &run)
+    uint64_t *status, uint64_t tile, uint32_t &run, uint32_t (&uoff)[kMaxRounds / 2])
 {
     constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
     constexpr int kRounds = kMaxRounds / 2;
     static_assert(kRounds % 2 == 0, "the size scan packs rounds in pairs");
+    // A unit's bytes: 4 + block_bytes(b, B - 1) <= 1 + 33 SPB.
+    static_assert(2u * K * (1u + 33u * SPB) <= (uint32_t)kOutCap, "a unit tile's bytes fit in ob");
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint32_t d = (uint32_t)tid / SPB, o = (uint32_t)tid % SPB;
     const uint32_t nu = nti / 2;  // units in the tile
@@ -956,7 +1051,6 @@ __device__ __forceinline__ void compress_units_fast(
         }
     }
     __syncthreads();
-    uint32_t off[kRounds];
     uint32_t acc = 0;
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
@@ -967,7 +1061,7 @@ __device__ __forceinline__ void compress_units_fast(
             before += w < wv ? t : 0u;
             rt += t;
         }
-        off[r] = acc + before + ex[r];
+        uoff[r] = acc + before + ex[r];
         acc += rt;
     }
     run = acc;
@@ -976,71 +1070,109 @@ __device__ __forceinline__ void compress_units_fast(
     for (int r = 0; r < kRounds; ++r) {
         if (r * K + d >= nu) continue;
         if (o == 0) {
-            if (INDEX) s_ioff[2 * (r * K + d)] = off[r];  // the unit's head item
-            lds_st32(ob + off[r], (uint32_t)hq[r]);
-            ob[off[r] + 4] = (uint8_t)bw[r];
+            lds_st32(ob + uoff[r], (uint32_t)hq[r]);
+            ob[uoff[r] + 4] = (uint8_t)bw[r];
         }
         if (bw[r]) {
             if (last)
-                encode_octet8<true>(ob, off[r] + 4, o, B - 1, bw[r], mg[r], sg[r]);
+                encode_octet8<true>(ob, uoff[r] + 4, o, B - 1, bw[r], mg[r], sg[r]);
             else
-                encode_octet8(ob, off[r] + 4, o, B - 1, bw[r], mg[r], sg[r]);
+                encode_octet8(ob, uoff[r] + 4, o, B - 1, bw[r], mg[r], sg[r]);
         }
     }
 }
 
-// A tile is g.R rounds of K items.  Each thread owns one octet per round and
-// loads it straight from HBM one round ahead of its use.  Rounds are encoded
-// into the LDS buffer ob (kOutCap bytes) as long as they fit; the first round
-// that does not fit and all later ones are encoded again after the tile's
-// offset is known.  One decoupled look-back per tile places the tile; tiles
-// are ordered by blockIdx.x (each XCD dispatches its workgroups in index
-// order, and every wait is bounded).
+// A tile is g.R rounds of K items.  Simple and unit tiles take the fast
+// paths above; every other tile (chunk heads, short blocks, B not a multiple
+// of 8) is encoded one round at a time, each thread owning one octet per
+// round, loaded one round ahead of its use.  Rounds are encoded into the LDS
+// buffer ob (kOutCap bytes) as long as they fit; the first round that does
+// not fit and all later ones are encoded again after the tile's offset is
+// known.  One decoupled look-back per tile places the tile; tiles are ordered
+// by blockIdx.x (each XCD dispatches its workgroups in index order, and every
+// wait is bounded).
 //
 // With INDEX the kernel also writes a side index beside the stream (the
 // stream itself is unchanged): the stream position of every block
 // (idx_pos[global block]) and, per tile, the q of the element before the
 // tile's first element (idx_carry[tile]), so that a decoder needs neither a
 // parse nor a scan across tiles.
-// LDS of one compress workgroup.
-template <bool INDEX>
+// LDS of one compress workgroup: the tile's encoded bytes (four wave regions
+// in the simple path) and the per-item scan arrays of the general path.
 struct CompressLds {
     __align__(16) uint8_t ob[kOutCap + 64];
-    uint32_t s_ioff[INDEX ? kWG * kMaxRounds : 1];
     uint32_t s_max[kWG];
     uint32_t s_off[kWG];
     uint32_t s_tot;
     uint32_t s_rbase[kMaxRounds + 1];
-    uint32_t s_head_t[kWG], s_head_off[kWG];
     uint32_t s_wtot[kMaxRounds * 4];
     uint64_t s_lb[8];
-    uint32_t s_nheads;
     int s_stop;
     uint64_t s_prefix;
 };
 
+// One round of a general tile: deltas of the thread's octet `cur` (loaded in
+// `in`), the block widths and item offsets of the round (s_off), and the
+// octet's bytes when the round fits in ob at `run`.  Returns the round's
+// bytes; ib = the thread's item's offset in the round.  Barriers inside.
+template <typename V>
+__device__ __forceinline__ uint32_t general_round(const Oct &cur, const OctIn<V> &in, V inv,
+                                                  uint32_t d, uint32_t o, uint32_t K,
+                                                  CompressLds &L, uint32_t run, bool encode,
+                                                  uint32_t &ib)
+{
+    const int tid = threadIdx.x;
+    uint32_t mg[8], sgn;
+    int32_t qh;
+    L.s_off[tid] = 0;  // the scan reads all kWG entries; items >= K contribute 0
+    const uint32_t omax = octet_deltas(cur, in, inv, mg, sgn, qh);
+    if (omax) atomicMax(&L.s_max[d], omax);
+    __syncthreads();
+    uint32_t b = 0, len = 0;
+    if (cur.live) {
+        if (cur.head) {
+            len = 4;
+        } else {
+            b = bit_width(L.s_max[d]);
+            len = block_bytes(b, cur.L);
+        }
+    }
+    if (o == 0 && d < K) L.s_off[d] = len;
+    __syncthreads();
+    L.s_max[tid] = 0;
+    if (tid < 64) {
+        const uint32_t a0 = L.s_off[4 * tid], a1 = L.s_off[4 * tid + 1];
+        const uint32_t a2 = L.s_off[4 * tid + 2], a3 = L.s_off[4 * tid + 3];
+        const uint32_t sum = a0 + a1 + a2 + a3;
+        const uint32_t incl = wave_incl_scan_u32(sum, tid);
+        const uint32_t ex = incl - sum;
+        L.s_off[4 * tid] = ex;
+        L.s_off[4 * tid + 1] = ex + a0;
+        L.s_off[4 * tid + 2] = ex + a0 + a1;
+        L.s_off[4 * tid + 3] = ex + a0 + a1 + a2;
+        if (tid == 63) L.s_tot = incl;
+    }
+    __syncthreads();
+    const uint32_t rt = L.s_tot;
+    ib = cur.live ? L.s_off[d] : 0u;
+    if (encode && run + rt <= (uint32_t)kOutCap) encode_octet(L.ob, run + ib, cur, o, b, mg, sgn, qh);
+    __syncthreads();  // s_off and s_tot are read before the next round writes them
+    return rt;
+}
+
 // One compress tile (all threads of the workgroup call it; every exit is
 // uniform, so a persistent workgroup can go on to its next tile).
 // EARLY_ERR: read the error word before the look-back (the single-field
-// kernel; the batched kernel, at its register limit, reads it after).
+// kernel; the batched kernel reads it after).
 template <typename V, bool INDEX, int SPB, bool EARLY_ERR = true>
 __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_eb,
                               V eb_host, const Geom &g, uint8_t *__restrict__ out,
                               uint64_t cap, uint64_t *__restrict__ status,
                               uint32_t *__restrict__ ctl, uint64_t *__restrict__ d_size,
                               uint64_t *__restrict__ idx_pos, uint32_t *__restrict__ idx_carry,
-                              uint64_t tile, CompressLds<INDEX> &L)
+                              uint64_t tile, CompressLds &L)
 {
-    uint8_t *ob = L.ob;
-    uint32_t *s_ioff = L.s_ioff, *s_max = L.s_max, *s_off = L.s_off, *s_rbase = L.s_rbase;
-    uint32_t *s_head_t = L.s_head_t, *s_head_off = L.s_head_off, *s_wtot = L.s_wtot;
-    uint64_t *s_lb = L.s_lb;
-    uint32_t &s_tot = L.s_tot, &s_nheads = L.s_nheads;
-    uint64_t &s_prefix = L.s_prefix;
-
-    const int tid = threadIdx.x;
-    s_max[tid] = 0;
-    if (tid == 0) s_nheads = 0;
+    const int tid = threadIdx.x, lane = __lane_id();
     const V eb = d_eb ? *d_eb : eb_host;
     // 1 / eb in the value type: FP32 division for float (SZp/src/szp_float.cc:185),
     // FP64 for double (SZp/src/szp_double.cc:185).
@@ -1052,110 +1184,69 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     uint32_t ct;
     uint64_t cj;
     locate(g, tI0, ct, cj);
-    const uint32_t t_first = ct;
-    const uint64_t j_first = cj;
-    const uint32_t nti_all = (uint32_t)(tIN - tI0);
-    const bool simple = simple_tile(g, ct, cj, nti_all);
-    const bool units = SPB && unit_tile(g, ct, cj, nti_all);
-    // The fast path writes the carry from data it loads anyway.
-    if (INDEX && tid == 0 && !(SPB && simple))
-        idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
-    uint32_t nit;
-    __syncthreads();
+    const uint32_t nti = (uint32_t)(tIN - tI0);
+    const bool simple = SPB && simple_tile(g, ct, cj, nti);
+    const bool units = SPB && !simple && unit_tile(g, ct, cj, nti);
 
-    uint64_t run = 0;   // tile-relative byte offset of the current round
-    uint32_t nbuf = 0;  // rounds held in ob
-    bool deferring = false;
-    bool published = false;
-    if (SPB && simple) {
-        compress_simple_fast<V, (SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
-                                                     s_wtot, s_rbase, s_ioff, status, tile, run,
-                                                     nbuf, idx_carry);
-        published = true;
+    uint32_t run = 0;       // the tile's bytes
+    // Simple path (per wave).
+    uint32_t qbase = 0, nfit = 0, fitbytes = 0, sbase[5] = {0, 0, 0, 0, 0},
+             boff[4] = {0, 0, 0, 0};
+    // Unit path: unit offsets in the tile (lanes o == 0).
+    uint32_t uoff[kMaxRounds / 2] = {0, 0};
+    // General path: rounds held in ob, and each round's item offset in the
+    // tile for the thread that starts the item (o == 0).
+    uint32_t nbuf = 0;
+    uint32_t ioff[kMaxRounds] = {0, 0, 0, 0};
+    if (simple) {
+        compress_simple_waves<V, (SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti, L.ob,
+                                                         L.s_wtot, status, tile, idx_carry, run,
+                                                         qbase, sbase, nfit, fitbytes, boff);
     } else if (units) {
-        compress_units_fast<V, (SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti_all, ob,
-                                                      s_wtot, s_ioff, status, tile, run);
+        if (INDEX && tid == 0)
+            idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
+        compress_units_fast<V, (SPB ? SPB : 1)>(x, inv, item_elem(g, ct, cj), nti, L.ob, L.s_wtot,
+                                                status, tile, run, uoff);
         nbuf = g.R;  // the whole tile is in ob
-        published = true;
-        if (cj == 0 && tid == 0) {  // the chunk's first unit: its offset goes in the header
-            s_head_t[0] = ct;
-            s_head_off[0] = 0;
-            s_nheads = 1;
-        }
     } else {
-        // General tiles (chunk heads, short blocks): every round's octet is
-        // loaded before the first is used, so the tile pays one memory
-        // latency and publishes its size about as early as a simple tile.
-        Oct oc[kMaxRounds];
-        OctIn<V> oi[kMaxRounds];
+        if (INDEX && tid == 0)
+            idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
+        L.s_max[tid] = 0;
+        __syncthreads();
+        // Round r + 1's octet is loaded while round r is encoded.
+        uint32_t t = ct;
+        uint64_t j = cj;
+        Oct cur{0, 0, 0, 0, false, false, false};
+        OctIn<V> cin;
         {
-            uint32_t t = ct;
-            uint64_t j = cj;
-#pragma unroll
-            for (int r = 0; r < kMaxRounds; ++r) {
-                const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
-                oc[r] = Oct{0, 0, 0, 0, false, false, false};
-                if ((uint32_t)r < g.R && ri0 < tIN) {
-                    const uint32_t nr = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
-                    oc[r] = octet_of(g, t, j, nr, d, o);
-                    octet_load(x, oc[r], oi[r]);
-                    advance(g, t, j, nr);
-                }
-            }
+            const uint32_t nr = (uint32_t)min<uint64_t>(g.K, tIN - tI0);
+            cur = octet_of(g, t, j, nr, d, o);
+            octet_load(x, cur, cin);
+            advance(g, t, j, nr);
         }
+        bool fits = true;
 #pragma unroll
-        for (int r = 0; r < kMaxRounds; ++r) {
+        for (uint32_t r = 0; r < (uint32_t)kMaxRounds; ++r) {
             const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
-            if ((uint32_t)r >= g.R || ri0 >= tIN) break;
-            const Oct &cur = oc[r];
-            uint32_t mg[8], sgn;
-            int32_t qh;
-            s_off[tid] = 0;  // the scan reads all kWG entries; items >= K contribute 0
-            const uint32_t omax = octet_deltas(cur, oi[r], inv, mg, sgn, qh);
-            if (omax) atomicMax(&s_max[d], omax);
-            __syncthreads();
-            uint32_t b = 0, len = 0;
-            if (cur.live) {
-                if (cur.head) {
-                    len = 4;
-                } else {
-                    b = bit_width(s_max[d]);
-                    len = block_bytes(b, cur.L);
-                }
+            if (r >= g.R || ri0 >= tIN) break;
+            Oct nxt{0, 0, 0, 0, false, false, false};
+            OctIn<V> nin;
+            const uint64_t rn0 = ri0 + g.K;
+            if (r + 1 < g.R && rn0 < tIN) {
+                const uint32_t nr = (uint32_t)min<uint64_t>(g.K, tIN - rn0);
+                nxt = octet_of(g, t, j, nr, d, o);
+                octet_load(x, nxt, nin);
+                advance(g, t, j, nr);
             }
-            if (o == 0 && d < g.K) s_off[d] = len;
-            __syncthreads();
-            s_max[tid] = 0;
-            if (tid < 64) {
-                const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
-                const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
-                const uint32_t sum = a0 + a1 + a2 + a3;
-                const uint32_t incl = wave_incl_scan_u32(sum, tid);
-                const uint32_t ex = incl - sum;
-                s_off[4 * tid] = ex;
-                s_off[4 * tid + 1] = ex + a0;
-                s_off[4 * tid + 2] = ex + a0 + a1;
-                s_off[4 * tid + 3] = ex + a0 + a1 + a2;
-                if (tid == 63) s_tot = incl;
-            }
-            __syncthreads();
-            const uint32_t rt = s_tot;
-            const uint32_t ib = cur.live ? s_off[d] : 0;
-            if (cur.first && o == 0) {
-                const uint32_t k = atomicAdd(&s_nheads, 1u);
-                s_head_t[k] = cur.t;
-                s_head_off[k] = (uint32_t)run + ib;
-            }
-            if (tid == 0) s_rbase[r] = (uint32_t)run;
-            if (INDEX && o == 0 && cur.live) s_ioff[r * g.K + d] = (uint32_t)run + ib;
-            if (!deferring && run + rt <= (uint64_t)kOutCap) {
-                encode_octet(ob, run + ib, cur, o, b, mg, sgn, qh);
-                nbuf = r + 1;
-            } else {
-                deferring = true;
-            }
+            if (tid == 0) L.s_rbase[r] = run;
+            uint32_t ib;
+            const uint32_t rt = general_round<V>(cur, cin, inv, d, o, g.K, L, run, fits, ib);
+            if (o == 0 && cur.live) ioff[r] = run + ib;
+            fits = fits && run + rt <= (uint32_t)kOutCap;
+            if (fits) nbuf = r + 1;
             run += rt;
-            __syncthreads();
+            cur = nxt;
+            cin = nin;
         }
     }
 
@@ -1165,11 +1256,12 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     // writes stay within the capacity check below).
     const uint32_t err0 = (EARLY_ERR && tid == 0) ? ld_ctl(&ctl[1]) : 0u;
     {
-        const uint64_t p = lookback_sum_wg(status, tile, 0, run, ctl, published, s_lb);
-        if (tid == 0) s_prefix = p;
+        const uint64_t p = lookback_sum_wg(status, tile, 0, run, ctl, simple || units, L.s_lb);
+        if (tid == 0) L.s_prefix = p;
     }
     __syncthreads();
-    const uint64_t G0 = g.body + s_prefix;
+    const uint64_t prefix = L.s_prefix;
+    const uint64_t G0 = g.body + prefix;
     if (tIN == g.nitems && tid == 0) *d_size = G0 + run;
     if (tid == 0) {
         L.s_stop = G0 + run > cap || ((EARLY_ERR ? err0 : ld_ctl(&ctl[1])) & kErrTimeout);
@@ -1177,29 +1269,6 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     }
     __syncthreads();
     if (L.s_stop) return;
-    if (INDEX) {
-        const uint32_t nti = (uint32_t)(tIN - tI0);
-        if (simple) {
-            // Blocks cj .. cj + nti - 1 of one chunk: consecutive index entries.
-            uint64_t *ip = idx_pos + (uint64_t)t_first * g.nb0 + (j_first - 1);
-            for (uint32_t k = tid; k < nti; k += kWG) ip[k] = G0 + s_ioff[k];
-        } else {
-            for (uint32_t k = tid; k < nti; k += kWG) {
-                uint32_t t = t_first;
-                uint64_t j = j_first;
-                advance(g, t, j, k);
-                if (item_indexed(g, j))
-                    idx_pos[(uint64_t)t * g.nb0 + item_block(g, j)] = G0 + s_ioff[k];
-            }
-        }
-    }
-    // Chunk offsets relative to the body (SZp/src/szp_float.cc:280-288).
-    for (uint32_t k = tid; k < s_nheads; k += kWG) {
-        const uint64_t off = s_prefix + s_head_off[k];
-        uint32_t *h = reinterpret_cast<uint32_t *>(out + g.es + 8ull * s_head_t[k]);
-        h[0] = (uint32_t)off;
-        h[1] = (uint32_t)(off >> 32);
-    }
     if (tile == 0) {
         if (tid == 0) put_eb(out, eb);
         // Empty chunks (n < T) all start at offset 0.
@@ -1211,70 +1280,106 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
             }
         }
     }
+
+    if (simple) {
+        constexpr uint32_t SP = SPB ? SPB : 1;
+        constexpr uint32_t K = 256 / SP, KS = 64 / SP;
+        const uint32_t wv = (uint32_t)tid >> 6;
+        const uint32_t ow = (uint32_t)lane % SP, b0 = wv * K + (uint32_t)lane / SP;
+        const uint64_t Gq = G0 + qbase;
+        if (INDEX && ow == 0) {
+            // Blocks cj .. cj + nti - 1 of chunk ct: consecutive index entries.
+            uint64_t *ip = idx_pos + (uint64_t)ct * g.nb0 + (cj - 1);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (b0 + (uint32_t)s * KS < nti) ip[b0 + s * KS] = Gq + boff[s];
+        }
+        uint8_t *reg = L.ob + wv * kWaveCap;
+        copy_out_wave(out, Gq, fitbytes, reg, (uint32_t)lane);
+        if (nfit < 4) {
+            const uint64_t Et = item_elem(g, ct, cj);
+#pragma unroll
+            for (uint32_t s = 0; s < 4; ++s) {
+                if (s < nfit) continue;  // uniform in the wave
+                simple_step_again<V, SP>(x, inv, Et, nti, s, reg, boff[s] - sbase[s]);
+                copy_out_wave(out, Gq + sbase[s], sbase[s + 1] - sbase[s], reg, (uint32_t)lane);
+            }
+        }
+        return;
+    }
+    if (units) {
+        constexpr uint32_t SP = SPB ? SPB : 1;
+        constexpr uint32_t K = 256 / SP;
+        if (o == 0) {
+#pragma unroll
+            for (int r = 0; r < kMaxRounds / 2; ++r) {
+                if (r * K + d >= nti / 2) continue;
+                if (INDEX) idx_pos[(uint64_t)ct * g.nb0 + cj / 2 + r * K + d] = G0 + uoff[r];
+            }
+        }
+        // The chunk's first unit: its offset goes in the header
+        // (SZp/src/szp_float.cc:280-288).
+        if (cj == 0 && tid == 0) {
+            uint32_t *h = reinterpret_cast<uint32_t *>(out + g.es + 8ull * ct);
+            h[0] = (uint32_t)prefix;
+            h[1] = (uint32_t)(prefix >> 32);
+        }
+        copy_out_bytes(out, G0, run, L.ob, tid);
+        return;
+    }
+    // General tile: index entries of its blocks (units in fastMode 2) and the
+    // header offsets of the chunks it starts (SZp/src/szp_float.cc:280-288).
+    if (o == 0) {
+#pragma unroll
+        for (uint32_t r = 0; r < (uint32_t)kMaxRounds; ++r) {
+            if (r >= g.R || (uint64_t)r * g.K + d >= nti || d >= g.K) break;
+            uint32_t t = ct;
+            uint64_t j = cj;
+            advance(g, t, j, r * g.K + d);
+            if (INDEX && item_indexed(g, j))
+                idx_pos[(uint64_t)t * g.nb0 + item_block(g, j)] = G0 + ioff[r];
+            if (j == 0) {
+                const uint64_t off = prefix + ioff[r];
+                uint32_t *h = reinterpret_cast<uint32_t *>(out + g.es + 8ull * t);
+                h[0] = (uint32_t)off;
+                h[1] = (uint32_t)(off >> 32);
+            }
+        }
+    }
     const bool all_buffered = tI0 + (uint64_t)nbuf * g.K >= tIN;
-    const uint32_t buffered = all_buffered ? (uint32_t)run : s_rbase[nbuf];
-    copy_out_bytes(out, G0, buffered, ob, tid);
+    const uint32_t buffered = all_buffered ? run : L.s_rbase[nbuf];
+    copy_out_bytes(out, G0, buffered, L.ob, tid);
     // Rounds that did not fit: encode again and write them one at a time.
     for (uint32_t r = nbuf; r < g.R; ++r) {
         const uint64_t ri0 = tI0 + (uint64_t)r * g.K;
         if (ri0 >= tIN) break;
-        nit = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
+        const uint32_t nit = (uint32_t)min<uint64_t>(g.K, tIN - ri0);
         uint32_t rt0;
         uint64_t rj0;
         locate(g, ri0, rt0, rj0);
         const Oct s = octet_of(g, rt0, rj0, nit, d, o);
         OctIn<V> in;
         octet_load(x, s, in);
-        uint32_t mg[8], sgn;
-        int32_t qh;
-        __syncthreads();
-        s_off[tid] = 0;
-        const uint32_t omax = octet_deltas(s, in, inv, mg, sgn, qh);
-        if (omax) atomicMax(&s_max[d], omax);
-        __syncthreads();
-        uint32_t b = 0, len = 0;
-        if (s.live) {
-            if (s.head) {
-                len = 4;
-            } else {
-                b = bit_width(s_max[d]);
-                len = block_bytes(b, s.L);
-            }
-        }
-        if (o == 0 && d < g.K) s_off[d] = len;
-        __syncthreads();
-        s_max[tid] = 0;
-        if (tid < 64) {
-            const uint32_t a0 = s_off[4 * tid], a1 = s_off[4 * tid + 1];
-            const uint32_t a2 = s_off[4 * tid + 2], a3 = s_off[4 * tid + 3];
-            const uint32_t sum = a0 + a1 + a2 + a3;
-            const uint32_t incl = wave_incl_scan_u32(sum, tid);
-            const uint32_t ex = incl - sum;
-            s_off[4 * tid] = ex;
-            s_off[4 * tid + 1] = ex + a0;
-            s_off[4 * tid + 2] = ex + a0 + a1;
-            s_off[4 * tid + 3] = ex + a0 + a1 + a2;
-            if (tid == 63) s_tot = incl;
-        }
-        __syncthreads();
-        encode_octet(ob, s.live ? s_off[d] : 0, s, o, b, mg, sgn, qh);
-        __syncthreads();
+        __syncthreads();  // ob is free (the previous copy has read it)
+        uint32_t ib;
+        const uint32_t rt = general_round<V>(s, in, inv, d, o, g.K, L, 0, true, ib);
         // The round was sized once already; a different size here would be a
         // defect, so it is reported instead of written.
-        const uint64_t rend = (ri0 + g.K < tIN) ? (uint64_t)s_rbase[r + 1] : run;
-        if ((uint64_t)s_rbase[r] + s_tot != rend || s_tot > (uint32_t)kOutCap) {
+        const uint32_t rend = (ri0 + g.K < tIN) ? L.s_rbase[r + 1] : run;
+        if (L.s_rbase[r] + rt != rend || rt > (uint32_t)kOutCap) {
             if (tid == 0) raise_error(ctl, kErrCapacity);
             return;
         }
-        copy_out_bytes(out, G0 + s_rbase[r], s_tot, ob, tid);
+        copy_out_bytes(out, G0 + L.s_rbase[r], rt, L.ob, tid);
     }
 }
 
-// Float fields: five workgroups per CU (at most 96 VGPRs; the LDS of
-// CompressLds, kOutCap, is sized to match).  Double fields keep more
-// registers (their octets are twice as wide) and run fewer workgroups.
+// Float fields: seven workgroups per CU (at most 72 VGPRs, no spills; the
+// LDS of CompressLds is under 20 KiB); at 64 VGPRs the simple path spilled.
+// Double fields keep more registers (their octets are twice as wide) and run
+// fewer workgroups.
 template <typename V, bool INDEX, int SPB>
-__global__ __launch_bounds__(kWG, sizeof(V) == 4 ? 5 : 1) void k_compress(const V *__restrict__ x,
+__global__ __launch_bounds__(kWG, sizeof(V) == 4 ? 7 : 1) void k_compress(const V *__restrict__ x,
                                                   const V *__restrict__ d_eb, V eb_host,
                                                   Geom g, uint8_t *__restrict__ out, uint64_t cap,
                                                   uint64_t *__restrict__ status,
@@ -1284,7 +1389,7 @@ __global__ __launch_bounds__(kWG, sizeof(V) == 4 ? 5 : 1) void k_compress(const 
                                                   uint32_t *__restrict__ idx_carry,
                                                   TileOrder ord)
 {
-    __shared__ CompressLds<INDEX> L;
+    __shared__ CompressLds L;
     compress_tile<V, INDEX, SPB>(x, d_eb, eb_host, g, out, cap, status, ctl, d_size, idx_pos,
                                  idx_carry, tile_of_block(ord, blockIdx.x), L);
 }
@@ -1681,7 +1786,12 @@ template <bool NT, typename V>
 __device__ __forceinline__ void store_octet_out(V *p, const V y[8])
 {
     if (NT) {
-        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        // Octets start at any 4-byte boundary (blocks start at a chunk's
+        // element lo + 1): a vector type of 4-byte alignment keeps the access
+        // well defined.  (16-byte-aligned non-temporal vectors shifted over
+        // DPP were measured slower and wrote more: 1.83x the output bytes
+        // against 1.28x, 355 against 285 us on a 512^3 decode.)
+        typedef uint32_t u4 __attribute__((ext_vector_type(4), aligned(4)));
         constexpr int nv = (int)sizeof(V) / 2;  // 16-byte pieces
         u4 w[nv];
         __builtin_memcpy(w, y, 8 * sizeof(V));
@@ -1691,68 +1801,6 @@ __device__ __forceinline__ void store_octet_out(V *p, const V y[8])
     } else {
         __builtin_memcpy(p, y, 8 * sizeof(V));
     }
-}
-
-// Stores the eight float values y of elements e .. e + 7 of `out` (p = out + e)
-// with 16-byte-aligned vector stores only.  a = e mod 4 is the same in every
-// lane of a wave (octets start 8 elements apart), so lane l writes the aligned
-// vectors [e - a, e - a + 4) -- the previous lane's last a values (DPP
-// wave_shr:1) and its own first 4 - a -- and [e - a + 4, e - a + 8).  Lane 0
-// writes its own first 4 - a values alone and lane 63 its last a values alone,
-// with ordinary stores: the vector they share with the neighbouring wave is
-// completed by the two.  All 64 lanes must call it (the DPP reads every lane).
-// NT: the aligned vectors are non-temporal (misaligned non-temporal vectors
-// write partial lines: measured 1.28x the output bytes on a 512^3 decode).
-template <bool NT, int A>
-__device__ __forceinline__ void store_octet_aligned_a(float *p, const float y[8])
-{
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    const int lane = __lane_id();
-    float prev[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < A; ++k)
-        prev[k] = __int_as_float(__builtin_amdgcn_update_dpp(
-            0, __float_as_int(y[8 - A + k]), 0x138, 0xF, 0xF, false));  // wave_shr:1
-    f4 v1, v2;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        v1[k] = k < A ? prev[k] : y[k - A];
-        v2[k] = y[4 - A + k];
-    }
-    f4 *q1 = reinterpret_cast<f4 *>(p - A), *q2 = reinterpret_cast<f4 *>(p + 4 - A);
-    if (A == 0 || lane != 0) {
-        if (NT) __builtin_nontemporal_store(v1, q1); else *q1 = v1;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4 - A; ++k) p[k] = y[k];
-    }
-    if (NT) __builtin_nontemporal_store(v2, q2); else *q2 = v2;
-    if (A != 0 && lane == 63) {
-#pragma unroll
-        for (int k = 8 - A; k < 8; ++k) p[k] = y[k];
-    }
-}
-
-template <bool NT>
-__device__ __forceinline__ void store_octet_aligned(float *p, const float y[8])
-{
-    switch ((reinterpret_cast<uintptr_t>(p) >> 2) & 3) {
-    case 0: store_octet_aligned_a<NT, 0>(p, y); break;
-    case 1: store_octet_aligned_a<NT, 1>(p, y); break;
-    case 2: store_octet_aligned_a<NT, 2>(p, y); break;
-    default: store_octet_aligned_a<NT, 3>(p, y); break;
-    }
-}
-
-// The octet stores of the simple-tile decodes: floats through the aligned
-// form above; doubles (already 8-byte aligned) with ordinary stores.
-template <bool NT, typename V>
-__device__ __forceinline__ void store_octet_tile(V *p, const V y[8])
-{
-    if constexpr (sizeof(V) == 4)
-        store_octet_aligned<NT>(p, y);
-    else
-        store_octet_out<false>(p, y);
 }
 
 // Unpacks one slot (SZp/src/szpd_float.cc:81-110, SZp/src/szp_TypeManager.cc:436-944).
@@ -2036,7 +2084,7 @@ __device__ __forceinline__ bool decode_simple_fast(
             V y[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
-            store_octet_tile<true>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
+            store_octet_out<sizeof(V) == 4>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
         }
     }
     return true;
@@ -2802,18 +2850,23 @@ __device__ __forceinline__ uint64_t take_ticket(uint64_t *queue, uint32_t ticket
     __syncthreads();  // the previous ticket's tiles are done with LDS and s_t
     if (threadIdx.x == 0) *s_t = atomicAdd(reinterpret_cast<unsigned long long *>(queue), ticket);
     __syncthreads();
-    return *s_t;
+    // The same value in every lane: made uniform, so that the queue entry and
+    // its job's geometry are loaded into scalar registers.
+    const uint64_t t = *s_t;
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(t >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)t);
 }
 
-// Five workgroups per CU (at most 96 VGPRs, a few bytes spilled): measured
-// on 256 uneven fields, 1.20 -> 1.39 TB/s against three per CU unbounded.
+// Five workgroups per CU: in the persistent loop the compiler holds the tile
+// body's lane-derived values (hoisted out of the loop) across tiles, and at
+// 72 VGPRs (seven per CU) it spilled about 200 bytes per lane.
 template <bool INDEX, int SPB>
 __global__ __launch_bounds__(kWG, 5) void k_batch_compress(const BatchJob *__restrict__ jobs,
                                                         const QEntry *__restrict__ qmap,
                                                         uint64_t total, uint64_t *queue,
                                                         uint32_t ticket, uint32_t *ctl)
 {
-    __shared__ CompressLds<INDEX> L;
+    __shared__ CompressLds L;
     __shared__ uint64_t s_t;
     for (uint64_t t0 = take_ticket(queue, ticket, &s_t); t0 < total;
          t0 = take_ticket(queue, ticket, &s_t)) {
@@ -2823,7 +2876,7 @@ __global__ __launch_bounds__(kWG, 5) void k_batch_compress(const BatchJob *__res
             const BatchJob &J = jobs[e.job];
             const Geom g = J.g;
             compress_tile<float, INDEX, SPB, false>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl,
-                                             J.d_size, J.idx_pos, J.idx_carry, e.tile, L);
+                                                    J.d_size, J.idx_pos, J.idx_carry, e.tile, L);
             __syncthreads();
         }
     }

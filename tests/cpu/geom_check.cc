@@ -60,6 +60,23 @@ static void check_compress_tile(const Geom &g, uint64_t tile, std::vector<uint8_
                       "unit octet outside its unit");
             }
     }
+    if (simple && g.spb <= 32 && (g.spb & (g.spb - 1)) == 0) {
+        // The simple path's wave mapping: every live lane's octet is its
+        // block's octet l % spb, inside the field; idle lanes read the tile's
+        // first octet.
+        const uint64_t Et = item_elem(g, ct, cj);
+        CHECK(Et >= 1 && Et + 8 <= g.n, "simple tile's first octet out of field");
+        for (uint32_t w = 0; w < (uint32_t)kWG / 64; ++w)
+            for (uint32_t s = 0; s < 4; ++s)
+                for (uint32_t l = 0; l < 64; ++l) {
+                    const uint32_t b = simple_wave_block(g, w, s, l);
+                    if (b >= nti) continue;
+                    const uint32_t off = simple_wave_off(w, s, l);
+                    CHECK(off == simple_octet_off(g, b / g.K, b % g.K, l % g.spb),
+                          "wave mapping (tile %llu w %u s %u l %u)", (unsigned long long)tile, w, s, l);
+                    CHECK(Et + off + 8 <= g.n, "wave octet out of field");
+                }
+    }
     uint32_t rt = ct;
     uint64_t rj = cj;
     for (uint32_t r = 0; r < g.R; ++r) {
