@@ -53,6 +53,24 @@ int szp_amd_compress_dev(const float *d_data, uint64_t nbEle, int errBoundMode,
                          unsigned char *d_out, uint64_t outCapacity, uint64_t *d_outSize,
                          void *d_work, uint64_t workBytes, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): szp_amd_compress_dev with its workspace taken from a
+ * static device array of SZP_AMD_NOALLOC_SCRATCH_BYTES per device, held until
+ * the process ends.  Allocates nothing (the array's address is looked up on a
+ * device's first call).  A field whose compress workspace
+ * (szp_amd_compress_workspace_size) exceeds the array is refused with
+ * SZP_AMD_ERR_CAPACITY before any launch (about 8.5e9 elements when blockSize
+ * is a multiple of 8).
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device race, so order them (one stream).  Device-side errors (output
+ * capacity, wait timeout) make *d_outSize read UINT64_MAX. */
+#define SZP_AMD_NOALLOC_SCRATCH_BYTES (8u << 20)
+int szp_amd_compress_dev_noalloc(const float *d_data, uint64_t nbEle, int errBoundMode,
+                                 double absErrBound, double relBoundRatio, int blockSize,
+                                 int nbThreads, unsigned char *d_out, uint64_t outCapacity,
+                                 uint64_t *d_outSize, void *stream);
+
 /* Device form of szp_decompress(SZP_NONRANDOMACCESS, SZ_FLOAT, ...)
  * (SZp/src/szp.cc:118-139).  streamBytes is the stream length or any upper
  * bound within the d_stream allocation. */

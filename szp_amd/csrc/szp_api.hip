@@ -423,6 +423,22 @@ int szp_amd_compress_dev(const float *d_data, uint64_t nbEle, int errBoundMode,
                            (hipStream_t)stream);
 }
 
+int szp_amd_compress_dev_noalloc(const float *d_data, uint64_t nbEle, int errBoundMode,
+                                 double absErrBound, double relBoundRatio, int blockSize,
+                                 int nbThreads, unsigned char *d_out, uint64_t outCapacity,
+                                 uint64_t *d_outSize, void *stream)
+{
+    const int rc = launch_compress_noalloc(d_data, nbEle, errBoundMode, absErrBound,
+                                           relBoundRatio, blockSize, nbThreads, d_out,
+                                           outCapacity, d_outSize, (hipStream_t)stream);
+    if (rc == SZP_AMD_ERR_CAPACITY)
+        set_error("szp_amd_compress_dev_noalloc: the field's workspace exceeds the "
+                  "8 MiB static device scratch (SZP_AMD_NOALLOC_SCRATCH_BYTES)");
+    else if (rc)
+        set_error("szp_amd_compress_dev_noalloc", rc);
+    return rc;
+}
+
 int szp_amd_decompress_dev(const unsigned char *d_stream, uint64_t streamBytes, uint64_t nbEle,
                            int blockSize, int nbThreads, float *d_out, void *d_work,
                            uint64_t workBytes, void *stream)
@@ -485,6 +501,7 @@ int szp_amd_decompress_dev_typed(int fastMode, int dataType, const unsigned char
 }
 
 static_assert(sizeof(szp_amd::BatchPlan) <= sizeof(szp_amd_batch), "batch plan size");
+static_assert(SZP_AMD_NOALLOC_SCRATCH_BYTES == szp_amd::kNoallocScratchBytes, "scratch size");
 static_assert(sizeof(szp_amd_field) == sizeof(szp_amd::BatchField), "field layout");
 
 uint64_t szp_amd_batch_workspace_size(const szp_amd_field *fields, int numFields, int blockSize,

@@ -33,6 +33,15 @@ int launch_compress(int dtype, const void *d_x, uint64_t n, int err_mode, double
 
 uint64_t index_bytes(uint64_t n, int B, int T, int mode = 1);
 
+// launch_compress (float, fastMode 1) with its workspace in a static device
+// array of kNoallocScratchBytes per device: allocates nothing.  Returns kCap
+// (2) when the field's workspace exceeds the array.  A device-side error
+// makes *d_size read ~0.  Calls that overlap on one device race.
+constexpr uint64_t kNoallocScratchBytes = 8ull << 20;
+int launch_compress_noalloc(const float *d_x, uint64_t n, int err_mode, double abs_eb,
+                            double rel, int B, int T, uint8_t *d_out, uint64_t cap,
+                            uint64_t *d_size, hipStream_t stream);
+
 int launch_decompress_indexed(int dtype, const uint8_t *d_stream, uint64_t stream_bytes,
                               const void *d_index, uint64_t idx_bytes, uint64_t n, int B, int T,
                               void *d_out, void *d_work, uint64_t work_bytes, hipStream_t stream,

@@ -3096,6 +3096,19 @@ class LookbackOrder {
             d_->ev = nullptr;
         if (d_->ev && d_->valid && d_->last != s) (void)hipStreamWaitEvent(s, d_->ev, 0);
     }
+    // Creates the device's event ahead of its first ordered launch (so that
+    // a later launch creates nothing).  Returns false on a HIP error.
+    static bool prepare(int dev)
+    {
+        if (dev < 0 || dev >= kDevs) return false;
+        Slot &d = slots()[dev];
+        std::lock_guard<std::mutex> lk(d.mu);
+        if (!d.ev && hipEventCreateWithFlags(&d.ev, hipEventDisableTiming) != hipSuccess) {
+            d.ev = nullptr;
+            return false;
+        }
+        return true;
+    }
     ~LookbackOrder()
     {
         if (!d_) return;
@@ -3260,6 +3273,58 @@ int launch_compress(int dtype, const void *d_x, uint64_t n, int err_mode, double
                                          abs_eb, rel, B, T, d_out, cap, d_size, d_work,
                                          work_bytes, stream, d_index, idx_bytes);
     return kArg;
+}
+
+// ---- compress with no allocation ----
+//
+// The workspace of launch_compress_noalloc is a file-scope device array, one
+// per device (each device holds its own copy until the process ends).  In host
+// code `scratch` names a host-side stand-in, not a device address: host code
+// uses only the address hipGetSymbolAddress returns for the current device
+// (noalloc_scratch), looked up once per device on its first call.
+static __device__ __align__(256) uint8_t scratch[kNoallocScratchBytes];
+
+// A device-side error of the call (ctl[1] != 0) makes its size read ~0.
+__global__ void k_size_status(const uint32_t *ctl, uint64_t *d_size)
+{
+    if (threadIdx.x == 0 && ctl[1]) *d_size = ~0ull;
+}
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch[64];
+}  // namespace
+
+// The current device's scratch address (set up on its first call), or null.
+static void *noalloc_scratch()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        if (!LookbackOrder::prepare(dev)) return nullptr;
+        g_scratch[dev] = p;
+    }
+    return g_scratch[dev];
+}
+
+int launch_compress_noalloc(const float *d_x, uint64_t n, int err_mode, double abs_eb,
+                            double rel, int B, int T, uint8_t *d_out, uint64_t cap,
+                            uint64_t *d_size, hipStream_t stream)
+{
+    const uint64_t need = compress_workspace_bytes(n, B, T, 1);
+    if (!need) return kArg;
+    if (need > kNoallocScratchBytes) return kCap;
+    void *ws = noalloc_scratch();
+    if (!ws) return kHip;
+    const int rc = launch_compress(0, d_x, n, err_mode, abs_eb, rel, B, T, d_out, cap, d_size, ws,
+                                   kNoallocScratchBytes, stream);
+    if (rc) return rc;
+    SZP_LAUNCH("k_size_status", k_size_status, dim3(1), dim3(64), 0, stream,
+               reinterpret_cast<const uint32_t *>(ws), d_size);
+    return hipGetLastError() == hipSuccess ? kOk : kHip;
 }
 
 template <typename V>
