@@ -1803,6 +1803,38 @@ __device__ __forceinline__ void store_octet_out(V *p, const V y[8])
     }
 }
 
+// Writes tot bytes (a multiple of 4) from LDS src to dst[0, tot) with
+// non-temporal stores: 16-byte stores for the aligned middle, each store
+// instruction of a wave covering 1 KiB of whole lines, and 4-byte stores for
+// the ragged ends, which share lines with the neighbouring tiles.
+__device__ __forceinline__ void copy_out_nt(uint8_t *__restrict__ dst, uint32_t tot, const uint8_t *src,
+                                            int tid)
+{
+    const uintptr_t G0 = (uintptr_t)dst, G1 = G0 + tot;
+    const uintptr_t a0 = (G0 + 15) & ~(uintptr_t)15, a1 = G1 & ~(uintptr_t)15;
+    const uint32_t head = (uint32_t)(a0 - G0) / 4, tail = (uint32_t)(G1 - a1) / 4;
+    if ((uint32_t)tid < head) {
+        uint32_t w;
+        __builtin_memcpy(&w, src + 4 * tid, 4);
+        __builtin_nontemporal_store(w, reinterpret_cast<uint32_t *>(dst) + tid);
+    } else if ((uint32_t)tid >= 4 && (uint32_t)tid < 4 + tail) {
+        const uint32_t k = (uint32_t)(a1 - G0) / 4 + (uint32_t)tid - 4;
+        uint32_t w;
+        __builtin_memcpy(&w, src + 4 * k, 4);
+        __builtin_nontemporal_store(w, reinterpret_cast<uint32_t *>(dst) + k);
+    }
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const uint32_t nv = (uint32_t)((a1 - a0) >> 4);
+    const uint8_t *s = src + (a0 - G0);
+    u4 *d = reinterpret_cast<u4 *>(dst + (a0 - G0));
+#pragma unroll 4
+    for (uint32_t v = tid; v < nv; v += kWG) {
+        u4 val;
+        __builtin_memcpy(&val, s + 16 * v, 16);  // any 4-byte LDS offset
+        __builtin_nontemporal_store(val, d + v);
+    }
+}
+
 // Unpacks one slot (SZp/src/szpd_float.cc:81-110, SZp/src/szp_TypeManager.cc:436-944).
 // The slot's item starts at buf[lp]; lend bounds the bytes present (reads may
 // run up to 32 bytes past it, inside the buffer's slack).  dl[i] = inclusive
@@ -2072,6 +2104,38 @@ __device__ __forceinline__ bool decode_simple_fast(
     }
     const uint64_t Et = item_elem(g, ct, cj);
     uint32_t base = carry;
+    if (sizeof(V) == 4) {
+        // Float: two rounds at a time through LDS (buf is free: every slot
+        // was unpacked before the barrier above), then whole-line
+        // non-temporal stores.  A round spans 2048 values (8 KiB).
+        static_assert(2 * 2048 * 4 <= kDecTileBytes, "two rounds of values fit the tile buffer");
+#pragma unroll
+        for (int r = 0; r < kMaxRounds; ++r) {
+            uint32_t before = base;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) before += w < wv ? s_wsum[r * 4 + w] : 0u;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) base += s_wsum[r * 4 + w];
+            if (r % 2 == 0) __syncthreads();  // buf: unpacked, or the previous pair written out
+            if ((uint32_t)r * K < nti) {
+                const uint32_t q0 = before + ex[r];
+                V y[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
+                __builtin_memcpy(buf + (r % 2) * 8192 + 32 * tid, y, 32);
+            }
+            if (r % 2 == 1 || r + 1 == kMaxRounds) {
+                const int r0 = r - r % 2;
+                const uint32_t nr = min<uint32_t>((uint32_t)(r - r0 + 1),
+                                                  (uint32_t)max<int64_t>(0, (int64_t)nti / K - r0));
+                __syncthreads();
+                if (nr)
+                    copy_out_nt(reinterpret_cast<uint8_t *>(out + Et + (uint64_t)r0 * 2048u), nr * 8192u,
+                                buf, tid);
+            }
+        }
+        return true;
+    }
 #pragma unroll
     for (int r = 0; r < kMaxRounds; ++r) {
         uint32_t before = base;
@@ -2084,7 +2148,7 @@ __device__ __forceinline__ bool decode_simple_fast(
             V y[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) y[i] = (V)(int32_t)(q0 + dl[r][i]) * eb;
-            store_octet_out<sizeof(V) == 4>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
+            store_octet_out<false>(out + Et + (uint64_t)r * 2048u + 8u * (uint32_t)tid, y);
         }
     }
     return true;
