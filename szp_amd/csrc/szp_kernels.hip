@@ -278,6 +278,131 @@ __device__ uint64_t lookback_sum_wg(uint64_t *st, uint64_t tile, uint64_t first,
     return excl;
 }
 
+// ---- compress look-back over tiles and groups of tiles ----
+//
+// With 1 500+ tiles in flight the nearest tile whose inclusive prefix is
+// published lies several hundred tiles back (measured: ~460 at the median on
+// a 512^3 field), and a look-back over single tiles then needs several
+// dependent steps of kWG statuses.  The compress kernels therefore also keep
+// one status word per group of kLbGroup consecutive tiles.  A tile status
+// keeps the tile's own byte count in its low bits in both states, so the
+// group's last tile, which reads every earlier status of its group anyway,
+// publishes the group's total (flag kFlagAgg) during its look-back, and its
+// inclusive prefix as the group's (flag kFlagIncl) at the end.  A tile reads
+// the earlier tiles of its group (wave 0) and 192 earlier groups (waves 1-3)
+// in one step; group totals do not wait for any look-back, so that step
+// almost always reaches an inclusive group.  Group words sit one 128-byte
+// line apart so that the polling of the few open groups is spread over
+// memory channels.
+//
+// Tile status: [flag 2][inclusive prefix 40][tile bytes 22]; group status:
+// [flag 2][value 62].
+constexpr uint32_t kLbGroup = 64;       // tiles per look-back group
+constexpr uint32_t kGroupStride = 16;   // u64 words between group statuses
+constexpr uint32_t kAggBits = 22;
+constexpr uint64_t kAggMask = (1ull << kAggBits) - 1;
+constexpr uint64_t kInclMask = (1ull << (62 - kAggBits)) - 1;
+// A tile holds at most kMaxRounds * 2048 values; each costs at most 4 head
+// bytes or 1/8 + 4 block bytes plus a width byte per block.
+static_assert((uint64_t)kMaxRounds * 2048 * 10 <= kAggMask, "a tile's byte count fits its field");
+constexpr uint64_t kMaxCompressStream = kInclMask;  // streams beyond 1 TiB are refused
+
+SZP_HD uint64_t lb_groups(uint64_t ntiles) { return (ntiles + kLbGroup - 1) / kLbGroup; }
+
+// Publishes a compress tile's byte count (tile 0: also its inclusive prefix).
+__device__ __forceinline__ void lookback_publish_c(uint64_t *st, uint64_t tile, uint64_t agg)
+{
+    if (__lane_id() == 0)
+        st_status(&st[tile], tile == 0 ? (kFlagIncl | (agg << kAggBits) | agg) : (kFlagAgg | agg));
+}
+
+// Sum of a wave's values up to and including its nearest inclusive one
+// (lanes ordered nearest first; v: each lane's value, incl: whether it is
+// inclusive), and whether it met one.
+__device__ __forceinline__ uint64_t wave_lookback_part(uint64_t v, bool incl, bool &found)
+{
+    const uint64_t m = __ballot(incl);
+    const int f = m ? __ffsll((long long)m) - 1 : 64;
+    found = m != 0;
+    return wave_sum_u64(__lane_id() <= f ? v : 0);
+}
+
+// Look-back of compress tile `tile` after lookback_publish_c (all threads
+// call it): returns its exclusive prefix in every thread, publishes its
+// inclusive prefix, and for a group's last tile the group's total and
+// inclusive prefix.  gst: group statuses (kGroupStride apart); s_lb: 8 u64
+// of LDS.
+__device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile, uint64_t agg,
+                                      uint32_t *ctl, uint64_t *s_lb)
+{
+    const int tid = threadIdx.x, wv = tid >> 6, lane = __lane_id();
+    const uint64_t g = tile / kLbGroup;
+    const uint32_t j = (uint32_t)(tile % kLbGroup);
+    const bool last = j + 1 == kLbGroup;
+    uint64_t excl = 0;
+    if (tile != 0) {
+        uint64_t v;
+        bool incl;
+        if (wv == 0) {
+            // The earlier tiles of this group, nearest first.
+            const bool in = (uint32_t)lane < j;
+            const uint64_t w = in ? wait_status(&st[tile - 1 - lane], ctl) : 0;
+            if (last) {
+                const uint64_t tot = wave_sum_u64(w & kAggMask) + agg;
+                if (lane == 0) st_status(&gst[g * kGroupStride], kFlagAgg | tot);
+            }
+            incl = in && (w >> 62) == 2;
+            v = incl ? ((w >> kAggBits) & kInclMask) : (w & kAggMask);
+        } else {
+            // Groups g - 1 .. g - 192, nearest first.
+            const int64_t gi = (int64_t)g - 1 - (int64_t)(tid - 64);
+            const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
+            incl = (w >> 62) == 2;
+            v = w & kValMask;
+        }
+        bool found;
+        const uint64_t part = wave_lookback_part(v, incl, found);
+        if (lane == 0) s_lb[wv] = (part << 1) | (found ? 1u : 0u);
+        __syncthreads();
+        bool done = false;
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) {
+            const uint64_t e = s_lb[w];
+            if (!done) {
+                excl += e >> 1;
+                done = e & 1;
+            }
+        }
+        // Further back, every wave reads groups (only when no group among
+        // the 192 before has its inclusive prefix yet).
+        int64_t hi = (int64_t)g - 1 - (kWG - 64);
+        while (!done) {  // uniform: every thread combines the same slots
+            const int64_t gi = hi - tid;
+            const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
+            bool fw;
+            const uint64_t pw = wave_lookback_part(w & kValMask, (w >> 62) == 2, fw);
+            __syncthreads();  // every thread has read the previous slots
+            if (lane == 0) s_lb[4 + wv] = (pw << 1) | (fw ? 1u : 0u);
+            __syncthreads();
+#pragma unroll
+            for (int w2 = 0; w2 < kWG / 64; ++w2) {
+                const uint64_t e = s_lb[4 + w2];
+                if (!done) {
+                    excl += e >> 1;
+                    done = e & 1;
+                }
+            }
+            hi -= kWG;
+        }
+    }
+    if (tid == 0) {
+        const uint64_t inc = excl + agg;
+        st_status(&st[tile], kFlagIncl | ((inc & kInclMask) << kAggBits) | agg);
+        if (last) st_status(&gst[g * kGroupStride], kFlagIncl | inc);
+    }
+    return excl;
+}
+
 // Copies tot bytes from LDS to out[G0, G0 + tot): 16-byte stores for the
 // aligned middle, byte stores for the ragged ends shared with neighbours.
 __device__ void copy_out_bytes(uint8_t *__restrict__ out, uint64_t G0, uint32_t tot,
@@ -921,7 +1046,7 @@ __device__ __forceinline__ void compress_simple_waves(
         qbase += w < wv ? t : 0u;
         run += t;
     }
-    if (tid < 64) lookback_publish(status, tile, 0, run);
+    if (tid < 64) lookback_publish_c(status, tile, run);
     uint8_t *reg = ob + (uint32_t)wv * kWaveCap;
     nfit = 0;
     fitbytes = 0;
@@ -1065,7 +1190,7 @@ __device__ __forceinline__ void compress_units_fast(
         acc += rt;
     }
     run = acc;
-    if (tid < 64) lookback_publish(status, tile, 0, acc);
+    if (tid < 64) lookback_publish_c(status, tile, acc);
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         if (r * K + d >= nu) continue;
@@ -1256,7 +1381,8 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     // writes stay within the capacity check below).
     const uint32_t err0 = (EARLY_ERR && tid == 0) ? ld_ctl(&ctl[1]) : 0u;
     {
-        const uint64_t p = lookback_sum_wg(status, tile, 0, run, ctl, simple || units, L.s_lb);
+        if (!(simple || units) && tid < 64) lookback_publish_c(status, tile, run);
+        const uint64_t p = lookback_compress(status, status + g.ntiles, tile, run, ctl, L.s_lb);
         if (tid == 0) L.s_prefix = p;
     }
     __syncthreads();
@@ -3017,6 +3143,13 @@ constexpr int kRangeParts = 2048;
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
+// Look-back words of a compress launch: a status per tile, then a status per
+// group of tiles, kGroupStride words apart.
+uint64_t compress_status_words(const Geom &g)
+{
+    return g.ntiles ? g.ntiles + kGroupStride * lb_groups(g.ntiles) : 1;
+}
+
 struct CompressWs {
     uint32_t *ctl;
     void *eb;     // the REL bound, float or double
@@ -3034,7 +3167,7 @@ uint64_t compress_ws_layout(const Geom &g, uint8_t *base, CompressWs *w)
     if (w) w->parts = base + off;
     off += align_up(sizeof(RangePart<double>) * kRangeParts);
     if (w) w->status = reinterpret_cast<uint64_t *>(base + off);
-    off += align_up(8 * (g.ntiles ? g.ntiles : 1));
+    off += align_up(8 * compress_status_words(g));
     return off;
 }
 
@@ -3285,7 +3418,7 @@ static int launch_compress_t(int mode, const V *d_x, uint64_t n, int err_mode, d
     CompressWs w;
     if (compress_ws_layout(g, reinterpret_cast<uint8_t *>(d_work), &w) > work_bytes) return kCap;
 
-    const uint64_t nst = g.ntiles ? g.ntiles : 1;
+    const uint64_t nst = compress_status_words(g);
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((nst + kWG - 1) / kWG, 4096);
     LookbackOrder order(stream);
     SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
@@ -3314,10 +3447,12 @@ static int launch_compress_t(int mode, const V *d_x, uint64_t n, int err_mode, d
         uint64_t *ipos;
         uint32_t *icarry;
         index_layout(g, reinterpret_cast<uint8_t *>(d_index), &ipos, &icarry);
-        launch_k_compress<V, true>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl,
+        launch_k_compress<V, true>(g, stream, d_x, d_eb, eb_host, d_out,
+                                   std::min<uint64_t>(cap, kMaxCompressStream), w.status, w.ctl,
                                    d_size, ipos, icarry);
     } else {
-        launch_k_compress<V, false>(g, stream, d_x, d_eb, eb_host, d_out, cap, w.status, w.ctl,
+        launch_k_compress<V, false>(g, stream, d_x, d_eb, eb_host, d_out,
+                                    std::min<uint64_t>(cap, kMaxCompressStream), w.status, w.ctl,
                                     d_size, nullptr, nullptr);
     }
     return hipGetLastError() == hipSuccess ? kOk : kHip;
@@ -3561,7 +3696,7 @@ bool batch_layout(const BatchField *f, int F, int B, int T, int decode, BatchLay
         if (f[k].n == 0 || !make_geom(f[k].n, B, T, &g)) return false;
         tiles += g.ntiles;
         if (!decode) {
-            nstat += g.ntiles;
+            nstat += compress_status_words(g);
             if (f[k].err_mode == 1) segs += (f[k].n + kBatchSeg - 1) / kBatchSeg;
         }
     }
@@ -3641,10 +3776,10 @@ int batch_prepare(const BatchField *f, int F, int B, int T, int decode, uint64_t
             if (fk.err_mode != 0 && fk.err_mode != 1) return kArg;
             J.x = reinterpret_cast<const float *>(fk.in);
             J.out = reinterpret_cast<uint8_t *>(fk.out);
-            J.cap = fk.out_cap;
+            J.cap = std::min<uint64_t>(fk.out_cap, kMaxCompressStream);
             J.d_size = d_sizes + k;
             J.status = reinterpret_cast<uint64_t *>(base + st_off);
-            st_off += 8 * g.ntiles;
+            st_off += 8 * compress_status_words(g);
             J.rel_mode = fk.err_mode == 1;
             J.eb = (float)fk.abs_eb;  // SZp/src/szp.cc:32
             J.rel = (float)fk.rel;
