@@ -285,15 +285,16 @@ __device__ uint64_t lookback_sum_wg(uint64_t *st, uint64_t tile, uint64_t first,
 // a 512^3 field), and a look-back over single tiles then needs several
 // dependent steps of kWG statuses.  The compress kernels therefore also keep
 // one status word per group of kLbGroup consecutive tiles.  A tile status
-// keeps the tile's own byte count in its low bits in both states, so the
-// group's last tile, which reads every earlier status of its group anyway,
-// publishes the group's total (flag kFlagAgg) during its look-back, and its
-// inclusive prefix as the group's (flag kFlagIncl) at the end.  A tile reads
-// the earlier tiles of its group (wave 0) and 192 earlier groups (waves 1-3)
-// in one step; group totals do not wait for any look-back, so that step
-// almost always reaches an inclusive group.  Group words sit one 128-byte
-// line apart so that the polling of the few open groups is spread over
-// memory channels.
+// keeps the tile's own byte count in its low bits in both states.  A group's
+// last tile publishes the group's total (flag kFlagAgg) right after its own
+// byte count, from its predecessors' statuses, and its inclusive prefix as
+// the group's (flag kFlagIncl) at the end of its look-back.  A tile reads the
+// earlier tiles of its group and of the group before (waves 0-1) and 128
+// earlier groups (waves 2-3) in one step: group totals do not wait for any
+// look-back, so that step almost always reaches an inclusive group.  Each
+// memory round trip costs ~2 us while the kernel streams, so one step
+// matters.  Group words sit one 128-byte line apart, spreading the polling
+// of the few open groups over memory channels.
 //
 // Tile status: [flag 2][inclusive prefix 40][tile bytes 22]; group status:
 // [flag 2][value 62].
@@ -309,11 +310,23 @@ constexpr uint64_t kMaxCompressStream = kInclMask;  // streams beyond 1 TiB are 
 
 SZP_HD uint64_t lb_groups(uint64_t ntiles) { return (ntiles + kLbGroup - 1) / kLbGroup; }
 
-// Publishes a compress tile's byte count (tile 0: also its inclusive prefix).
-__device__ __forceinline__ void lookback_publish_c(uint64_t *st, uint64_t tile, uint64_t agg)
+// Publishes a compress tile's byte count (tile 0: also its inclusive prefix);
+// wave 0 calls it.  A group's last tile then also publishes the group's total
+// from its predecessors' statuses (published earlier, as they started
+// earlier): the one wait here is a single round trip for one tile in
+// kLbGroup, and it makes the group's total available a whole encode pass
+// before the tile's own look-back.
+__device__ __forceinline__ void lookback_publish_c(uint64_t *st, uint64_t *gst, uint64_t tile,
+                                                   uint64_t agg, uint32_t *ctl)
 {
-    if (__lane_id() == 0)
+    const int lane = __lane_id();
+    if (lane == 0)
         st_status(&st[tile], tile == 0 ? (kFlagIncl | (agg << kAggBits) | agg) : (kFlagAgg | agg));
+    if (tile % kLbGroup == kLbGroup - 1) {
+        const uint64_t w = lane < (int)kLbGroup - 1 ? wait_status(&st[tile - 1 - lane], ctl) : 0;
+        const uint64_t tot = wave_sum_u64(w & kAggMask) + agg;
+        if (lane == 0) st_status(&gst[(tile / kLbGroup) * kGroupStride], kFlagAgg | tot);
+    }
 }
 
 // Sum of a wave's values up to and including its nearest inclusive one
@@ -328,9 +341,8 @@ __device__ __forceinline__ uint64_t wave_lookback_part(uint64_t v, bool incl, bo
 }
 
 // Look-back of compress tile `tile` after lookback_publish_c (all threads
-// call it): returns its exclusive prefix in every thread, publishes its
-// inclusive prefix, and for a group's last tile the group's total and
-// inclusive prefix.  gst: group statuses (kGroupStride apart); s_lb: 8 u64
+// call it): returns its exclusive prefix in every thread and publishes its
+// inclusive prefix, and for a group's last tile the group's.  gst: group statuses (kGroupStride apart); s_lb: 8 u64
 // of LDS.
 __device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile, uint64_t agg,
                                       uint32_t *ctl, uint64_t *s_lb)
@@ -343,19 +355,18 @@ __device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile
     if (tile != 0) {
         uint64_t v;
         bool incl;
-        if (wv == 0) {
-            // The earlier tiles of this group, nearest first.
-            const bool in = (uint32_t)lane < j;
-            const uint64_t w = in ? wait_status(&st[tile - 1 - lane], ctl) : 0;
-            if (last) {
-                const uint64_t tot = wave_sum_u64(w & kAggMask) + agg;
-                if (lane == 0) st_status(&gst[g * kGroupStride], kFlagAgg | tot);
-            }
+        if (wv < 2) {
+            // The earlier tiles of this group and of the one before, nearest
+            // first (the previous group's total comes one round trip after
+            // its last tile's byte count, and that tile may have started
+            // just before this one).
+            const bool in = (uint32_t)tid < j + (g ? kLbGroup : 0u);
+            const uint64_t w = in ? wait_status(&st[tile - 1 - tid], ctl) : 0;
             incl = in && (w >> 62) == 2;
             v = incl ? ((w >> kAggBits) & kInclMask) : (w & kAggMask);
         } else {
-            // Groups g - 1 .. g - 192, nearest first.
-            const int64_t gi = (int64_t)g - 1 - (int64_t)(tid - 64);
+            // Groups g - 2 .. g - 129, nearest first.
+            const int64_t gi = (int64_t)g - 2 - (int64_t)(tid - 128);
             const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
             incl = (w >> 62) == 2;
             v = w & kValMask;
@@ -373,9 +384,9 @@ __device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile
                 done = e & 1;
             }
         }
-        // Further back, every wave reads groups (only when no group among
-        // the 192 before has its inclusive prefix yet).
-        int64_t hi = (int64_t)g - 1 - (kWG - 64);
+        // Further back, every thread reads a group (only when no group
+        // among the 128 before has its inclusive prefix yet).
+        int64_t hi = (int64_t)g - 2 - (kWG - 128);
         while (!done) {  // uniform: every thread combines the same slots
             const int64_t gi = hi - tid;
             const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
@@ -988,7 +999,8 @@ __device__ __forceinline__ int32_t q_before(const int32_t q[8], int32_t qp)
 template <typename V, int SPB, bool INDEX>
 __device__ __forceinline__ void compress_simple_waves(
     const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob, uint32_t *s_wtot,
-    uint64_t *status, uint64_t tile, uint32_t *idx_carry, uint32_t &run, uint32_t &qbase,
+    uint64_t *status, uint64_t *gst, uint32_t *ctl, uint64_t tile, uint32_t *idx_carry,
+    uint32_t &run, uint32_t &qbase,
     uint32_t (&sbase)[5], uint32_t &nfit, uint32_t &fitbytes, uint32_t (&boff)[4])
 {
     constexpr uint32_t K = 256 / SPB, KS = 64 / SPB, B = 8 * SPB;
@@ -1046,7 +1058,7 @@ __device__ __forceinline__ void compress_simple_waves(
         qbase += w < wv ? t : 0u;
         run += t;
     }
-    if (tid < 64) lookback_publish_c(status, tile, run);
+    if (tid < 64) lookback_publish_c(status, gst, tile, run, ctl);
     uint8_t *reg = ob + (uint32_t)wv * kWaveCap;
     nfit = 0;
     fitbytes = 0;
@@ -1116,7 +1128,8 @@ __device__ __forceinline__ void simple_step_again(const V *__restrict__ x, V inv
 template <typename V, int SPB>
 __device__ __forceinline__ void compress_units_fast(
     const V *__restrict__ x, V inv, uint64_t Et, uint32_t nti, uint8_t *ob, uint32_t *s_wtot,
-    uint64_t *status, uint64_t tile, uint32_t &run, uint32_t (&uoff)[kMaxRounds / 2])
+    uint64_t *status, uint64_t *gst, uint32_t *ctl, uint64_t tile, uint32_t &run,
+    uint32_t (&uoff)[kMaxRounds / 2])
 {
     constexpr uint32_t K = 256 / SPB, B = 8 * SPB;
     constexpr int kRounds = kMaxRounds / 2;
@@ -1190,7 +1203,7 @@ __device__ __forceinline__ void compress_units_fast(
         acc += rt;
     }
     run = acc;
-    if (tid < 64) lookback_publish_c(status, tile, acc);
+    if (tid < 64) lookback_publish_c(status, gst, tile, acc, ctl);
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         if (r * K + d >= nu) continue;
@@ -1314,6 +1327,7 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     const bool units = SPB && !simple && unit_tile(g, ct, cj, nti);
 
     uint32_t run = 0;       // the tile's bytes
+    uint64_t *gst = status + g.ntiles;  // look-back group statuses
     // Simple path (per wave).
     uint32_t qbase = 0, nfit = 0, fitbytes = 0, sbase[5] = {0, 0, 0, 0, 0},
              boff[4] = {0, 0, 0, 0};
@@ -1325,13 +1339,14 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     uint32_t ioff[kMaxRounds] = {0, 0, 0, 0};
     if (simple) {
         compress_simple_waves<V, (SPB ? SPB : 1), INDEX>(x, inv, item_elem(g, ct, cj), nti, L.ob,
-                                                         L.s_wtot, status, tile, idx_carry, run,
+                                                         L.s_wtot, status, gst, ctl, tile,
+                                                         idx_carry, run,
                                                          qbase, sbase, nfit, fitbytes, boff);
     } else if (units) {
         if (INDEX && tid == 0)
             idx_carry[tile] = cj ? (uint32_t)quantize(x[item_elem(g, ct, cj) - 1], inv) : 0u;
         compress_units_fast<V, (SPB ? SPB : 1)>(x, inv, item_elem(g, ct, cj), nti, L.ob, L.s_wtot,
-                                                status, tile, run, uoff);
+                                                status, gst, ctl, tile, run, uoff);
         nbuf = g.R;  // the whole tile is in ob
     } else {
         if (INDEX && tid == 0)
@@ -1381,8 +1396,8 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     // writes stay within the capacity check below).
     const uint32_t err0 = (EARLY_ERR && tid == 0) ? ld_ctl(&ctl[1]) : 0u;
     {
-        if (!(simple || units) && tid < 64) lookback_publish_c(status, tile, run);
-        const uint64_t p = lookback_compress(status, status + g.ntiles, tile, run, ctl, L.s_lb);
+        if (!(simple || units) && tid < 64) lookback_publish_c(status, gst, tile, run, ctl);
+        const uint64_t p = lookback_compress(status, gst, tile, run, ctl, L.s_lb);
         if (tid == 0) L.s_prefix = p;
     }
     __syncthreads();
