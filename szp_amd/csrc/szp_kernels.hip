@@ -506,6 +506,7 @@ __device__ __forceinline__ void vec_minmax(const uint8_t *p, double &mn, double 
 // seg a multiple of 4 * kWG * kRangeUnroll elements (only the last segment
 // is ragged).
 constexpr int kRangeUnroll = 4;
+constexpr int kRangeParts = 2048;  // segments (workgroups) of the pass at most
 
 // Segments are taken from the field's end backwards (workgroup b reads
 // segment gridDim.x - 1 - b), with non-temporal loads.  Measured on a 512^3
@@ -516,10 +517,21 @@ constexpr int kRangeUnroll = 4;
 // first); with non-temporal loads it runs at 82 us after either, and the
 // compress kernel's loss of those cache hits (233 -> 255 us) is smaller than
 // the gain (compress + decompress step 0.530 -> 0.503 ms).
+//
+// The pass also clears the compress call's control and look-back words (the
+// compress kernel follows in stream order), which saves a launch.
 template <typename V, bool VEC>
 __global__ __launch_bounds__(kWG) void k_range_partial(const V *__restrict__ x, uint64_t n,
-                                                       uint64_t seg, RangePart<V> *part)
+                                                       uint64_t seg, RangePart<V> *part,
+                                                       uint32_t *ctl, uint64_t *status,
+                                                       uint64_t nstatus)
 {
+    {
+        const uint64_t per = (nstatus + gridDim.x - 1) / gridDim.x;
+        const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = min(b0 + per, nstatus);
+        for (uint64_t k = b0 + threadIdx.x; k < b1; k += kWG) status[k] = 0;
+        if (blockIdx.x == 0 && threadIdx.x < 8) ctl[threadIdx.x] = 0;
+    }
     constexpr uint32_t kPer = 16 / sizeof(V);  // values per 16-byte load
     const uint32_t sg = gridDim.x - 1 - blockIdx.x;
     const uint64_t s = (uint64_t)sg * seg;
@@ -561,11 +573,19 @@ __global__ __launch_bounds__(kWG) void k_range_final(const V *__restrict__ x,
                                                      const RangePart<V> *part, int nparts,
                                                      V relf, V *d_eb)
 {
+    // Every part is loaded before any is combined (one memory round trip).
+    constexpr int kPer = kRangeParts / kWG;
+    RangePart<V> p[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int k = (int)threadIdx.x + u * kWG;
+        p[u] = k < nparts ? part[k] : RangePart<V>{(V)__builtin_nan(""), (V)__builtin_nan("")};
+    }
     V mn = (V)__builtin_nan(""), mx = (V)__builtin_nan("");
-    for (int k = threadIdx.x; k < nparts; k += kWG) {
-        const RangePart<V> p = part[k];
-        mn = vmin(mn, p.mn);
-        mx = vmax(mx, p.mx);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        mn = vmin(mn, p[u].mn);
+        mx = vmax(mx, p[u].mx);
     }
     range_reduce_wg(mn, mx);
     if (threadIdx.x == 0) {
@@ -3154,7 +3174,6 @@ int profile_entry(int i, char *name, int cap, uint64_t *launches, double *total_
 namespace {
 
 constexpr uint64_t kAlign = 256;
-constexpr int kRangeParts = 2048;
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -3436,7 +3455,9 @@ static int launch_compress_t(int mode, const V *d_x, uint64_t n, int err_mode, d
     const uint64_t nst = compress_status_words(g);
     const uint32_t cgrid = (uint32_t)std::min<uint64_t>((nst + kWG - 1) / kWG, 4096);
     LookbackOrder order(stream);
-    SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
+    // REL: the range pass clears these words.
+    if (err_mode != 1)
+        SZP_LAUNCH("k_clear", k_clear, dim3(cgrid), dim3(kWG), 0, stream, w.ctl, w.status, nst);
     const V *d_eb = nullptr;
     // ABS: cast to the value type (float: SZp/src/szp.cc:32; double: :62).
     const V eb_host = (V)abs_eb;
@@ -3448,9 +3469,11 @@ static int launch_compress_t(int mode, const V *d_x, uint64_t n, int err_mode, d
         RangePart<V> *parts = reinterpret_cast<RangePart<V> *>(w.parts);
         V *d_ebw = reinterpret_cast<V *>(w.eb);
         if (vec)
-            SZP_LAUNCH("k_range_partial", (k_range_partial<V, true>), dim3(np), dim3(kWG), 0, stream, d_x, n, seg, parts);
+            SZP_LAUNCH("k_range_partial", (k_range_partial<V, true>), dim3(np), dim3(kWG), 0, stream,
+                       d_x, n, seg, parts, w.ctl, w.status, nst);
         else
-            SZP_LAUNCH("k_range_partial", (k_range_partial<V, false>), dim3(np), dim3(kWG), 0, stream, d_x, n, seg, parts);
+            SZP_LAUNCH("k_range_partial", (k_range_partial<V, false>), dim3(np), dim3(kWG), 0, stream,
+                       d_x, n, seg, parts, w.ctl, w.status, nst);
         SZP_LAUNCH("k_range_final", k_range_final<V>, dim3(1), dim3(kWG), 0, stream, d_x, parts, np,
                    (V)rel, d_ebw);
         d_eb = d_ebw;
