@@ -277,9 +277,54 @@ def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
                 best = lat
         return best
 
+    def timed_graph(g):
+        ms = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            g.replay()
+            e1.record(s)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return min(ms)
+
     pc = timed_plain(plain_compress)
     pd = timed_plain(plain_decompress)
     log("batched leg: plain calls timed")
+
+    # Host-free plain baseline: the same per-field calls captured in one
+    # graph, so that the comparison isolates the persistent kernel from the
+    # Python loop's launch cost.  Replays are checked against the eager
+    # streams and values.
+    def capture(fn):
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            fn()
+        torch.cuda.synchronize()
+        return g
+
+    def plain_compress_calls():
+        for c, x, o, ix in zip(codecs, xs, outs, idxs):
+            c.compress(x, *mode, out=o, index=ix)
+
+    def plain_decompress_calls():
+        for c, o, sz, x, d, ix in zip(codecs, outs, sizes, xs, decs, idxs):
+            c.decompress(o, sz, x.numel(), out=d, index=ix)
+
+    ref_streams = [o[:sz].clone() for o, sz in zip(outs, sizes)]
+    ref_decs = [d.clone() for d in decs]
+    gpc = capture(plain_compress_calls)
+    gpd = capture(plain_decompress_calls)
+    pgc = timed_graph(gpc)
+    pgd = timed_graph(gpd)
+    torch.cuda.synchronize()
+    plain_graph_same = (all(torch.equal(o[:sz], r) for o, sz, r in zip(outs, sizes, ref_streams))
+                        and all(torch.equal(d.view(torch.int32), r.view(torch.int32))
+                                for d, r in zip(decs, ref_decs)))
+    del gpc, gpd, ref_streams, ref_decs
+    log("batched leg: graphed plain calls timed")
 
     # Batched: one persistent launch for all fields, autotuned, in a graph.
     cb = sb.CompressBatch(xs, mode, block_size=B, threads=T, index=True)
@@ -300,18 +345,6 @@ def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
                    for k in range(len(xs)))
     gd = db.graph()
 
-    def timed_graph(g):
-        ms = []
-        for _ in range(reps):
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(s)
-            g.replay()
-            e1.record(s)
-            e1.synchronize()
-            ms.append(e0.elapsed_time(e1))
-        return min(ms)
-
     bc = timed_graph(gc)
     cb.check()
     bd = timed_graph(gd)
@@ -329,6 +362,11 @@ def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
         "plain_decompress_GBps": round(4 * N / (pd[-1] * 1e-3) / 1e9, 1),
         "batched_decompress_GBps": round(4 * N / (bd * 1e-3) / 1e9, 1),
         "decompress_speedup": round(pd[-1] / bd, 2),
+        "plain_graphed_compress_GBps": round(4 * N / (pgc * 1e-3) / 1e9, 1),
+        "plain_graphed_decompress_GBps": round(4 * N / (pgd * 1e-3) / 1e9, 1),
+        "compress_speedup_vs_graphed_plain": round(pgc / bc, 2),
+        "decompress_speedup_vs_graphed_plain": round(pgd / bd, 2),
+        "graphed_plain_replays_identical": bool(plain_graph_same),
         "plain_compress_latency_ms": {"p50": round(_pct(pc, 50), 3), "p99": round(_pct(pc, 99), 3)},
         "batched_compress_latency_ms": {"p50": round(bc, 3), "p99": round(bc, 3)},
         "plain_decompress_latency_ms": {"p50": round(_pct(pd, 50), 3), "p99": round(_pct(pd, 99), 3)},
@@ -569,18 +607,67 @@ def large_field_bench(torch, szp, B=64, T=16, rel=1e-4, seed=5, steps=3, cpu_che
         "error_bound": {"eb": eb, "max_err_over_eb": worst / eb, "strict_over_eb": strict,
                         "within_rounding_bound": ok},
     }
+    # Batched: the same field as the one job of a persistent launch (graph
+    # replay), against the plain call's stream and values.
+    from szp_amd import batch as sb
+    cb = sb.CompressBatch([x], (szp.REL, 0.0, rel), block_size=B, threads=T, index=True)
+    cb.launch()
+    cb.check()
+    bsz = cb.byte_sizes()[0]
+    gc = cb.graph()
+    bc_ms = _timed(torch, gc.replay, steps)
+    cb.check()
+    bstream = cb.stream(0, bsz)
+    db = sb.DecompressBatch([bstream], [bsz], [N], [cb.index_of(0)], block_size=B, threads=T)
+    db.launch()
+    db.check()
+    gd = db.graph()
+    bd_ms = _timed(torch, gd.replay, steps)
+    db.check()
+    res["batched"] = {
+        "compress_GBps": round(4 * N / (bc_ms * 1e-3) / 1e9, 1),
+        "compress_hbm_frac": round((8 * N + C) / (bc_ms * 1e-3) / 8e12, 4),
+        "decompress_GBps": round(4 * N / (bd_ms * 1e-3) / 1e9, 1),
+        "decompress_hbm_frac": round((C + 4 * N) / (bd_ms * 1e-3) / 8e12, 4),
+        "stream_byte_identical_to_plain": bool(bsz == C and torch.equal(bstream, out[:C])),
+        "decompress_bit_identical_to_plain": bool(torch.equal(db.outs[0].view(torch.int32),
+                                                              dec.view(torch.int32))),
+        "graph": "one hipGraph replay per launch (kernels only)",
+    }
+    del cb, db, gc, gd, bstream
+    torch.cuda.empty_cache()
+    log("large field: batched timed")
     if cpu_check:
-        # The oracle on the host (16 threads) for the same field: byte parity.
+        # The reference compiled from its sources (else the oracle) on the
+        # host, T threads, on the same field: byte and bit parity.
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
         xh = x.cpu().numpy()
-        t0 = time.perf_counter()
-        want = oracle_lib.compress(xh, szp.REL, 0.0, rel, B, T, workers=16)
-        tc = time.perf_counter() - t0
         got = out[:C].cpu().numpy()
-        res["stream_byte_identical_to_oracle"] = bool(want.size == C and np.array_equal(want, got))
-        res["oracle_compress_seconds_16_threads"] = round(tc, 2)
-        del xh, want, got
+        if oracle_lib.reference() is not None:
+            kind = "reference"
+            t0 = time.perf_counter()
+            want = oracle_lib.ref_compress(xh, szp.REL, 0.0, rel, B, T)
+            tc = time.perf_counter() - t0
+            same = bool(want.size == C and np.array_equal(want, got))
+            del xh
+            t0 = time.perf_counter()
+            wdec = oracle_lib.ref_decompress(want, N, B, T)
+            td = time.perf_counter() - t0
+            res["decompress_bit_identical_to_reference"] = bool(
+                np.array_equal(wdec.view(np.uint32), dec.cpu().numpy().view(np.uint32)))
+            res["reference_decompress_seconds"] = round(td, 2)
+            del wdec
+        else:
+            kind = "port"
+            t0 = time.perf_counter()
+            want = oracle_lib.compress(xh, szp.REL, 0.0, rel, B, T, workers=T)
+            tc = time.perf_counter() - t0
+            same = bool(want.size == C and np.array_equal(want, got))
+            del xh
+        res["cpu_check"] = {"kind": kind, "threads": T, "compress_seconds": round(tc, 2),
+                            "stream_byte_identical": same}
+        del want, got
     del x, out, dec, idx, codec
     torch.cuda.empty_cache()
     return res
