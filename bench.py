@@ -373,8 +373,10 @@ def batched_bench(torch, szp, count=256, rel=1e-3, B=128, T=16, seed=7, reps=5):
         "batched_decompress_latency_ms": {"p50": round(bd, 3), "p99": round(bd, 3)},
         "latency_note": "time from the start of the sequence to each field's completion; the batch "
                         "completes all fields together",
-        "autotune": {"compress": {"ticket": tune_c["ticket"], "grid": tune_c["grid"]},
-                     "decompress": {"ticket": tune_d["ticket"], "grid": tune_d["grid"]}},
+        "autotune": {k: {"form": t["form"], "ticket": t["ticket"],
+                         "grid": t["grid"] if t["form"] == "persistent" else "one per tile",
+                         "resident_grid": t["resident_grid"], "shapes_tried": len(t["trials"])}
+                     for k, t in (("compress", tune_c), ("decompress", tune_d))},
         "streams_byte_identical_to_plain": bool(same),
         "decompress_bit_identical_to_plain": bool(same_dec),
         "graph": "one hipGraph replay per batch (kernels only)",
@@ -611,6 +613,7 @@ def large_field_bench(torch, szp, B=64, T=16, rel=1e-4, seed=5, steps=3, cpu_che
     # replay), against the plain call's stream and values.
     from szp_amd import batch as sb
     cb = sb.CompressBatch([x], (szp.REL, 0.0, rel), block_size=B, threads=T, index=True)
+    tune_c = sb.autotune(cb)
     cb.launch()
     cb.check()
     bsz = cb.byte_sizes()[0]
@@ -619,6 +622,7 @@ def large_field_bench(torch, szp, B=64, T=16, rel=1e-4, seed=5, steps=3, cpu_che
     cb.check()
     bstream = cb.stream(0, bsz)
     db = sb.DecompressBatch([bstream], [bsz], [N], [cb.index_of(0)], block_size=B, threads=T)
+    tune_d = sb.autotune(db)
     db.launch()
     db.check()
     gd = db.graph()
@@ -633,6 +637,9 @@ def large_field_bench(torch, szp, B=64, T=16, rel=1e-4, seed=5, steps=3, cpu_che
         "decompress_bit_identical_to_plain": bool(torch.equal(db.outs[0].view(torch.int32),
                                                               dec.view(torch.int32))),
         "graph": "one hipGraph replay per launch (kernels only)",
+        "autotune": {k: {"form": t["form"], "ticket": t["ticket"],
+                         "grid": t["grid"] if t["form"] == "persistent" else "one per tile"}
+                     for k, t in (("compress", tune_c), ("decompress", tune_d))},
     }
     del cb, db, gc, gd, bstream
     torch.cuda.empty_cache()

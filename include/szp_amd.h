@@ -143,10 +143,17 @@ int szp_amd_batch_prepare(const szp_amd_field *fields, int numFields, int blockS
  * and replay it).  ticket: tiles a workgroup takes at a time (>= 1;
  * decompress only, compress takes one tile at a time because a tile waits for
  * the tiles before it to start); grid: persistent workgroups (0 = one per
- * resident slot).  Status as for the single-field calls:
- * szp_amd_check_status(d_work, stream). */
+ * resident slot), or, for compress only, SZP_AMD_BATCH_GRID_PER_TILE: one
+ * workgroup per tile, each taking one ticket.  Status as for the
+ * single-field calls: szp_amd_check_status(d_work, stream). */
+#define SZP_AMD_BATCH_GRID_PER_TILE 0xFFFFFFFFu
 int szp_amd_batch_launch(const szp_amd_batch *batch, uint32_t ticket, uint32_t grid,
                          void *stream);
+
+/* Workgroups of the batch's persistent kernel that fit on the current device
+ * at once (CUs x workgroups per CU by occupancy): the resident grid that
+ * grid = 0 launches, and the base of the autotuner's grid candidates. */
+uint32_t szp_amd_batch_resident_grid(const szp_amd_batch *batch);
 
 /* Synchronises `stream` and returns the status accumulated in d_work. */
 int szp_amd_check_status(void *d_work, void *stream);

@@ -31,6 +31,12 @@ lib.szp_amd_batch_prepare.argtypes = [ctypes.POINTER(Field), _i32, _i32, _i32, _
                                       _u64, ctypes.POINTER(_Plan)]
 lib.szp_amd_batch_launch.restype = _i32
 lib.szp_amd_batch_launch.argtypes = [ctypes.POINTER(_Plan), ctypes.c_uint32, ctypes.c_uint32, _vp]
+lib.szp_amd_batch_resident_grid.restype = ctypes.c_uint32
+lib.szp_amd_batch_resident_grid.argtypes = [ctypes.POINTER(_Plan)]
+
+# grid value of szp_amd_batch_launch for one compress workgroup per tile
+# (SZP_AMD_BATCH_GRID_PER_TILE in include/szp_amd.h).
+GRID_PER_TILE = 0xFFFFFFFF
 
 
 def _align(v, a=256):
@@ -71,6 +77,10 @@ class _Batch:
                                       _stream_ptr(self.torch, stream, self.device))
         if rc:
             raise SzpError("szp_amd_batch_launch", rc)
+
+    def resident_grid(self):
+        """Workgroups of the persistent kernel that fit the device at once."""
+        return int(lib.szp_amd_batch_resident_grid(ctypes.byref(self.plan)))
 
     def check(self, stream=None):
         rc = lib.szp_amd_check_status(self.work.data_ptr(),
@@ -187,32 +197,42 @@ def _time_launch(batch, ticket, grid, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def autotune(batch, tickets=(1, 2, 4, 8, 16), grid_fractions=(1.0, 0.75, 0.5), reps=3, key=None):
-    """Times the batch under each (ticket, grid) candidate and keeps the
-    fastest on the batch (and in a cache under `key`).  grid fractions are of
-    the resident grid (CUs x workgroups per CU); the workgroup size is fixed at
-    256 threads by the tile geometry; compress batches take one tile per
-    ticket, so only the grid is tuned for them.  Returns {"ticket", "grid",
-    "ms", "trials"}."""
-    if not batch.decode:
-        tickets = (1,)
+def autotune(batch, tickets=(1, 2, 4, 8, 16), grid_fractions=(1.0, 0.75, 0.5, 0.25), reps=3,
+             key=None):
+    """Times the batch under each launch shape and keeps the fastest on the
+    batch (and in a cache under `key`).  The shapes searched:
+
+    * form: for compress, the persistent kernel (workgroups loop over
+      tickets) or one workgroup per tile (GRID_PER_TILE: no loop, so the
+      kernel runs at the single-field kernel's occupancy);
+    * grid: the persistent kernel's workgroups, as fractions of the resident
+      grid (CUs x workgroups per CU, from the kernel's occupancy);
+    * ticket: tiles a workgroup takes at a time (decompress; a compress tile
+      waits for the tiles before it to start, so compress takes one).
+
+    Workgroup size (256 threads, four wave64s) and values per thread (one
+    octet of 8 per round) are fixed by the tile geometry, which maps a
+    round's 2048 values onto the workgroup; the stream does not depend on
+    any of the shapes.  Returns {"ticket", "grid", "form", "ms", "trials"}."""
     if key is not None and key in _TUNED:
         best = _TUNED[key]
         batch.ticket, batch.grid = best["ticket"], best["grid"]
         return best
-    torch = batch.torch
-    props = torch.cuda.get_device_properties(batch.device)
-    cus = props.multi_processor_count
+    resident = batch.resident_grid() or 1024
+    if not batch.decode:
+        tickets = (1,)
     trials = []
     for frac in grid_fractions:
-        # resident slots per CU: up to 4 with this LDS and register budget
-        grid = 0 if frac == 1.0 else max(1, int(cus * 4 * frac))
+        grid = 0 if frac == 1.0 else max(1, int(resident * frac))
         for t in tickets:
             ms = _time_launch(batch, t, grid, reps)
-            trials.append({"ticket": t, "grid": grid, "ms": ms})
+            trials.append({"ticket": t, "grid": grid, "form": "persistent", "ms": ms})
+    if not batch.decode:
+        ms = _time_launch(batch, 1, GRID_PER_TILE, reps)
+        trials.append({"ticket": 1, "grid": GRID_PER_TILE, "form": "per_tile", "ms": ms})
     best = min(trials, key=lambda r: r["ms"])
     batch.ticket, batch.grid = best["ticket"], best["grid"]
-    res = dict(best, trials=trials)
+    res = dict(best, trials=trials, resident_grid=resident)
     if key is not None:
         _TUNED[key] = res
     return res

@@ -502,6 +502,7 @@ int szp_amd_decompress_dev_typed(int fastMode, int dataType, const unsigned char
 
 static_assert(sizeof(szp_amd::BatchPlan) <= sizeof(szp_amd_batch), "batch plan size");
 static_assert(SZP_AMD_NOALLOC_SCRATCH_BYTES == szp_amd::kNoallocScratchBytes, "scratch size");
+static_assert(SZP_AMD_BATCH_GRID_PER_TILE == szp_amd::kBatchGridPerTile, "per-tile grid value");
 static_assert(sizeof(szp_amd_field) == sizeof(szp_amd::BatchField), "field layout");
 
 uint64_t szp_amd_batch_workspace_size(const szp_amd_field *fields, int numFields, int blockSize,
@@ -532,6 +533,14 @@ int szp_amd_batch_launch(const szp_amd_batch *batch, uint32_t ticket, uint32_t g
     szp_amd::BatchPlan plan;
     memcpy(&plan, batch, sizeof(plan));
     return batch_launch(plan, ticket, grid, (hipStream_t)stream);
+}
+
+uint32_t szp_amd_batch_resident_grid(const szp_amd_batch *batch)
+{
+    if (!batch) return 0;
+    szp_amd::BatchPlan plan;
+    memcpy(&plan, batch, sizeof(plan));
+    return batch_resident_grid(plan);
 }
 
 int szp_amd_check_status(void *d_work, void *stream)

@@ -38,6 +38,8 @@ uint64_t index_bytes(uint64_t n, int B, int T, int mode = 1);
 // (2) when the field's workspace exceeds the array.  A device-side error
 // makes *d_size read ~0.  Calls that overlap on one device race.
 constexpr uint64_t kNoallocScratchBytes = 8ull << 20;
+// batch_launch's grid value for one compress workgroup per tile.
+constexpr uint32_t kBatchGridPerTile = 0xFFFFFFFFu;
 int launch_compress_noalloc(const float *d_x, uint64_t n, int err_mode, double abs_eb,
                             double rel, int B, int T, uint8_t *d_out, uint64_t cap,
                             uint64_t *d_size, hipStream_t stream);
@@ -85,6 +87,8 @@ int batch_prepare(const BatchField *f, int F, int B, int T, int decode, uint64_t
 // entries a workgroup takes at a time; grid: persistent workgroups (0 = one
 // per resident slot).
 int batch_launch(const BatchPlan &plan, uint32_t ticket, uint32_t grid, hipStream_t stream);
+// Workgroups of the batch's persistent kernel that fit the device at once.
+uint32_t batch_resident_grid(const BatchPlan &plan);
 
 // Device address of the workspace's error word (u32).
 uint32_t *workspace_error_word(void *d_work);

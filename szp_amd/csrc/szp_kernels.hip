@@ -3107,6 +3107,31 @@ __global__ __launch_bounds__(kWG, 5) void k_batch_compress(const BatchJob *__res
     }
 }
 
+// One tile per workgroup (grid = the batch's tiles): a workgroup takes one
+// ticket and exits, so no loop holds hoisted values across tiles and the
+// kernel keeps the single-field kernel's seven workgroups per CU.  The
+// ticket still orders the tiles: a workgroup takes it once resident, and a
+// tile waits only for lower tickets, taken by workgroups already running.
+template <bool INDEX, int SPB>
+__global__ __launch_bounds__(kWG, 7) void k_batch_compress_once(const BatchJob *__restrict__ jobs,
+                                                             const QEntry *__restrict__ qmap,
+                                                             uint64_t total, uint64_t *queue,
+                                                             uint32_t *ctl)
+{
+    __shared__ CompressLds L;
+    __shared__ uint64_t s_t;
+    if (threadIdx.x == 0) s_t = atomicAdd(reinterpret_cast<unsigned long long *>(queue), 1ull);
+    __syncthreads();
+    const uint64_t q = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(s_t >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)s_t);
+    if (q >= total) return;
+    const QEntry e = qmap[q];
+    const BatchJob &J = jobs[e.job];
+    const Geom g = J.g;
+    compress_tile<float, INDEX, SPB, false>(J.x, &J.eb, 0.f, g, J.out, J.cap, J.status, ctl,
+                                            J.d_size, J.idx_pos, J.idx_carry, e.tile, L);
+}
+
 template <int SPB>
 __global__ __launch_bounds__(kWG) void k_batch_decode(const BatchJob *__restrict__ jobs,
                                                       const QEntry *__restrict__ qmap,
@@ -3867,6 +3892,7 @@ int batch_prepare(const BatchField *f, int F, int B, int T, int decode, uint64_t
 int batch_launch(const BatchPlan &P, uint32_t ticket, uint32_t grid, hipStream_t stream)
 {
     if (!P.work || !P.F || ticket == 0) return kArg;
+    if (grid == kBatchGridPerTile && (P.decode || P.tiles > 0x7fffffffu)) return kArg;
     // A compress tile waits until every lower tile of its field has started;
     // a workgroup starts the later tiles of a ticket only after the earlier
     // ones finish, so tickets of several compress tiles would chain the
@@ -3909,9 +3935,14 @@ int batch_launch(const BatchPlan &P, uint32_t ticket, uint32_t grid, hipStream_t
     }
 #define SZP_BATCH_COMPRESS_CASE(I, S)                                                            \
     case S: {                                                                                    \
-        const uint32_t gr = grid ? grid : resident_grid(k_batch_compress<I, S>);                 \
-        SZP_LAUNCH("k_batch_compress", (k_batch_compress<I, S>), dim3(gr), block, 0, stream,     \
-                   jobs, qmap, P.tiles, status, ticket, ctl);                                    \
+        if (grid == kBatchGridPerTile) {                                                         \
+            SZP_LAUNCH("k_batch_compress", (k_batch_compress_once<I, S>), dim3(P.tiles), block,  \
+                       0, stream, jobs, qmap, P.tiles, status, ctl);                             \
+        } else {                                                                                 \
+            const uint32_t gr = grid ? grid : resident_grid(k_batch_compress<I, S>);             \
+            SZP_LAUNCH("k_batch_compress", (k_batch_compress<I, S>), dim3(gr), block, 0, stream, \
+                       jobs, qmap, P.tiles, status, ticket, ctl);                                \
+        }                                                                                        \
         break;                                                                                   \
     }
     if (P.index) {
@@ -3937,6 +3968,14 @@ int batch_launch(const BatchPlan &P, uint32_t ticket, uint32_t grid, hipStream_t
     }
 #undef SZP_BATCH_COMPRESS_CASE
     return hipGetLastError() == hipSuccess ? kOk : kHip;
+}
+
+uint32_t batch_resident_grid(const BatchPlan &P)
+{
+    // Occupancy does not depend on the template arguments' SPB beyond
+    // registers, which the SPB = 16 forms bound.
+    if (P.decode) return resident_grid(k_batch_decode<16>);
+    return P.index ? resident_grid(k_batch_compress<true, 16>) : resident_grid(k_batch_compress<false, 16>);
 }
 
 }  // namespace szp_amd

@@ -96,6 +96,61 @@ def test_batch_launch_shapes_agree(env):
             assert np.array_equal(cb.stream(k, ref_sizes[k]).cpu().numpy(), ref[k]), (ticket, grid, k)
 
 
+@pytest.mark.parametrize("B,T", [(128, 16), (32, 3), (2048, 2), (9, 4)])
+def test_batch_compress_per_tile_form(env, B, T):
+    """One workgroup per tile (GRID_PER_TILE) gives the oracle's bytes, with
+    and without the side index, twice in a row."""
+    torch, szp, batch = env
+    xs_h, xs, modes = _inputs(torch)
+    want = [oracle_lib.compress(x, *modes[k], B, T) for k, x in enumerate(xs_h)]
+    for index in (True, False):
+        say("per-tile form B=%d T=%d index=%s" % (B, T, index))
+        cb = batch.CompressBatch(xs, modes, block_size=B, threads=T, index=index)
+        for rep in range(2):
+            cb.out.zero_()
+            cb.launch(grid=batch.GRID_PER_TILE)
+            cb.check()
+            sizes = cb.byte_sizes()
+            for k in range(len(xs)):
+                assert sizes[k] == want[k].size, (index, rep, k)
+                assert np.array_equal(cb.stream(k, sizes[k]).cpu().numpy(), want[k]), (index, rep, k)
+
+
+def test_batch_autotune_shapes(env):
+    """The autotuner tries both compress forms and several grids from the
+    resident grid, and its choice gives the same bytes."""
+    torch, szp, batch = env
+    xs_h, xs, modes = _inputs(torch)
+    cb = batch.CompressBatch(xs, modes, block_size=128, threads=16, index=True)
+    cb.launch()
+    cb.check()
+    sizes = cb.byte_sizes()
+    ref = [cb.stream(k, sizes[k]).cpu().numpy().copy() for k in range(len(xs))]
+    tune = batch.autotune(cb, reps=1)
+    say("autotune", {k: v for k, v in tune.items() if k != "trials"})
+    forms = {t["form"] for t in tune["trials"]}
+    assert forms == {"persistent", "per_tile"}
+    assert tune["resident_grid"] >= 256
+    assert len({t["grid"] for t in tune["trials"]}) >= 4
+    cb.out.zero_()
+    cb.launch()  # the chosen shape
+    cb.check()
+    assert cb.byte_sizes() == sizes
+    for k in range(len(xs)):
+        assert np.array_equal(cb.stream(k, sizes[k]).cpu().numpy(), ref[k]), k
+    db = batch.DecompressBatch([cb.stream(k, sizes[k]) for k in range(len(xs))], sizes,
+                               [x.size for x in xs_h], [cb.index_of(k) for k in range(len(xs))],
+                               block_size=128, threads=16)
+    tuned = batch.autotune(db, reps=1)
+    assert {t["form"] for t in tuned["trials"]} == {"persistent"}
+    assert len({t["ticket"] for t in tuned["trials"]}) >= 4
+    db.launch()
+    db.check()
+    for k, x in enumerate(xs_h):
+        want = oracle_lib.decompress(cb.stream(k, sizes[k]).cpu().numpy(), x.size, 128, 16)
+        assert np.array_equal(db.outs[k].cpu().numpy().view(np.uint32), want.view(np.uint32)), k
+
+
 def test_batch_graph_replays(env):
     """A captured batch replays to the eager result, with eager work between
     replays (the graph records kernels only)."""
