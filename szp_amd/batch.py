@@ -67,7 +67,9 @@ class _Batch:
                                        ctypes.byref(self.plan))
         if rc:
             raise SzpError("szp_amd_batch_prepare", rc)
-        self.ticket, self.grid = (2 if decode else 1), 0
+        # Defaults: decompress two tiles per ticket on the resident grid;
+        # compress one workgroup per tile.
+        self.ticket, self.grid = (2, 0) if decode else (1, GRID_PER_TILE)
 
     def launch(self, stream=None, ticket=None, grid=None):
         """Enqueues the batch (kernels only) on `stream`."""

@@ -986,6 +986,22 @@ __device__ __forceinline__ uint32_t octet_mags(const int32_t q[8], int32_t prev,
     return omax;
 }
 
+// The largest magnitude of an octet's deltas (octet_mags' return value) from
+// the deltas' maximum and minimum: |d| peaks at the largest positive or the
+// most negative delta (INT32_MIN gives 0x80000000, as octet_mags does).
+__device__ __forceinline__ uint32_t octet_max_mag(const int32_t q[8], int32_t prev)
+{
+    int32_t hi = 0, lo = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int32_t dd = (int32_t)((uint32_t)q[i] - (uint32_t)prev);
+        prev = q[i];
+        hi = max(hi, dd);
+        lo = min(lo, dd);
+    }
+    return max((uint32_t)hi, 0u - (uint32_t)lo);
+}
+
 // The q of the element before the lane's octet: the previous lane's last q
 // (DPP wave_shr:1); lane 0 keeps qp, the q of its own preceding element.
 __device__ __forceinline__ int32_t q_before(const int32_t q[8], int32_t qp)
@@ -1055,8 +1071,7 @@ __device__ __forceinline__ void compress_simple_waves(
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const bool live = b0 + (uint32_t)s * KS < nti;
-        uint32_t mg[8], sg;
-        const uint32_t omax = item_max<SPB>(octet_mags(q[s], q_before(q[s], qp[s]), mg, sg));
+        const uint32_t omax = item_max<SPB>(octet_max_mag(q[s], q_before(q[s], qp[s])));
         bw[s] = live ? bit_width(omax) : 0u;
         const uint32_t sz = live ? (bw[s] ? 1u + SPB * (bw[s] + 1u) : 1u) : 0u;  // block_bytes(bw, B)
         const uint32_t v = o == 0 ? sz : 0u;

@@ -20,7 +20,7 @@ import collections
 import time
 
 from . import REL, compress_bound, lib
-from .batch import CompressBatch, DecompressBatch, Field
+from .batch import GRID_PER_TILE, CompressBatch, DecompressBatch, Field
 
 
 class Request:
@@ -47,18 +47,20 @@ class Server:
 
     budget: bytes of HBM the in-flight requests may hold (outputs, side
     indexes, batch workspaces); inputs are resident and not counted.
-    streams: batches in flight at once.
+    streams: batches in flight at once.  ticket, grid: the decompress
+    batches' launch shape; compress_grid: the compress batches' (one
+    workgroup per tile by default, the faster form on mixed sizes).
     """
 
     def __init__(self, block_size=128, threads=16, budget=64 << 30, streams=3, ticket=2, grid=0,
-                 device=None):
+                 compress_grid=GRID_PER_TILE, device=None):
         import torch
         self.torch = torch
         self.B, self.T = int(block_size), int(threads)
         self.budget = int(budget)
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
-        self.ticket, self.grid = ticket, grid
+        self.ticket, self.grid, self.compress_grid = ticket, grid, compress_grid
         self.used = 0
         self.peak = 0
         self.batches = 0
@@ -106,7 +108,7 @@ class Server:
             if comp:
                 cb = CompressBatch([r.x for r in comp], [r.mode for r in comp], block_size=self.B,
                                    threads=self.T, index=True, device=self.device)
-                cb.launch(stream=stream, ticket=self.ticket, grid=self.grid)
+                cb.launch(stream=stream, ticket=1, grid=self.compress_grid)
                 keep.append(("c", cb, comp))
             if deco:
                 db = DecompressBatch([r.stream for r in deco], [r.nbytes for r in deco],
