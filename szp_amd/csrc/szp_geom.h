@@ -337,6 +337,23 @@ SZP_HD void bm_set_range(uint32_t *bm, uint32_t p, uint32_t k)
     if (!k) return;
     for (uint32_t w = p >> 5; w <= (p + k - 1) >> 5; ++w) bm[w] |= bm_mask(w, p, k);
 }
+// How the chain walks below set bitmap bits: OR a mask into a word.  The
+// kernel passes an LDS atomic OR (its single-lane walks then do not wait for
+// each word's old value); the host emulation a plain OR.
+struct BmOr {
+    SZP_HD void operator()(uint32_t *w, uint32_t m) const { *w |= m; }
+};
+template <typename OR>
+SZP_HD void bm_set_by(uint32_t *bm, uint32_t p, OR op)
+{
+    op(&bm[p >> 5], 1u << (p & 31));
+}
+template <typename OR>
+SZP_HD void bm_set_range_by(uint32_t *bm, uint32_t p, uint32_t k, OR op)
+{
+    if (!k) return;
+    for (uint32_t w = p >> 5; w <= (p + k - 1) >> 5; ++w) op(&bm[w], bm_mask(w, p, k));
+}
 SZP_HD void bm_clear_range(uint32_t *bm, uint32_t p, uint32_t k)
 {
     if (!k) return;
@@ -426,24 +443,25 @@ SZP_HD bool parse_chain_leaves(const uint8_t *tb, uint32_t c, uint32_t v, uint32
 
 // Marks the block starts of the chain from c (which leaves the sub-tile) in
 // spec; returns their count.
+template <typename OR = BmOr>
 SZP_HD uint32_t parse_mark_chain(const uint8_t *tb, uint32_t *spec, uint32_t c, uint32_t v,
-                                 uint32_t B, uint32_t mode = 1)
+                                 uint32_t B, uint32_t mode = 1, OR op = OR())
 {
     uint32_t p = c, n = 0;
     while (p < v) {
         uint32_t len;
         const uint32_t b = parse_step(tb, p, B, mode, len);
         if (mode == 2) {
-            bm_set(spec, p);
+            bm_set_by(spec, p, op);
             ++n;
             p += len;
         } else if (b == 0) {
             const uint32_t k = zero_run(tb, p, v);
-            bm_set_range(spec, p, k);
+            bm_set_range_by(spec, p, k, op);
             n += k;
             p += k;
         } else {
-            bm_set(spec, p);
+            bm_set_by(spec, p, op);
             ++n;
             p += len;
         }
@@ -476,8 +494,9 @@ SZP_HD uint32_t parse_cand_walk(const uint8_t *tb, const uint32_t *spec, uint32_
 // True chain from p in [u, v): marks its blocks before it joins spec in pre.
 // Returns 1 merged (p = the join position), 2 broken (width byte > 32 at p),
 // 0 left the sub-tile (p >= v).  npre counts the marked blocks.
+template <typename OR = BmOr>
 SZP_HD int parse_true_walk(const uint8_t *tb, const uint32_t *spec, uint32_t *pre, uint32_t &p,
-                           uint32_t v, uint32_t B, uint32_t &npre, uint32_t mode = 1)
+                           uint32_t v, uint32_t B, uint32_t &npre, uint32_t mode = 1, OR op = OR())
 {
     npre = 0;
     while (p < v) {
@@ -488,7 +507,7 @@ SZP_HD int parse_true_walk(const uint8_t *tb, const uint32_t *spec, uint32_t *pr
         if (b == 0 && mode == 1) {
             const uint32_t k = zero_run(tb, p, v);
             const uint32_t q = bm_first(spec, p, k);
-            bm_set_range(pre, p, q - p);
+            bm_set_range_by(pre, p, q - p, op);
             npre += q - p;
             if (q < p + k) {
                 p = q;
@@ -496,7 +515,7 @@ SZP_HD int parse_true_walk(const uint8_t *tb, const uint32_t *spec, uint32_t *pr
             }
             p += k;
         } else {
-            bm_set(pre, p);
+            bm_set_by(pre, p, op);
             ++npre;
             p += len;
         }

@@ -1679,6 +1679,12 @@ __global__ void k_dsetup(const uint8_t *__restrict__ st, uint64_t len, Geom g, u
     ttab[g.T] = tiles;
 }
 
+// LDS bitmap bits set with ds_or: a walk's next width-byte read does not wait
+// for the word's old value.
+struct BmAtomicOr {
+    __device__ void operator()(uint32_t *w, uint32_t m) const { atomicOr(w, m); }
+};
+
 // Recovers the start of every SZp block of a stream that carries no index.
 // Parse tile i of chunk t covers bytes [s, te) of the chunk's block area and
 // is split into four sub-tiles of U bytes, one per wave.  The "entry" of a
@@ -1775,7 +1781,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
             if (m) {
                 const int w = __ffsll((long long)m) - 1;  // the lowest candidate that leaves
                 uint32_t n = 0;
-                if (lane == w) n = parse_mark_chain(tb, spec, c, v, B, mode);
+                if (lane == w) n = parse_mark_chain(tb, spec, c, v, B, mode, BmAtomicOr());
                 cnt = __builtin_amdgcn_readlane(n, w);
                 ex = __builtin_amdgcn_readlane(e, w);
                 ok = true;
@@ -1836,7 +1842,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
             s_broken[w] = broken;
             if (uw >= span || broken || e >= s + vw) continue;  // no block starts here
             uint32_t p = (uint32_t)(e - s), npre = 0;
-            const int res = parse_true_walk(tb, spec, pre, p, vw, B, npre, mode);
+            const int res = parse_true_walk(tb, spec, pre, p, vw, B, npre, mode, BmAtomicOr());
             s_npre[w] = npre;
             if (res == 1) {
                 s_merged[w] = 1;
