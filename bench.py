@@ -431,21 +431,27 @@ def serving_bench(torch, szp, rate=2000.0, count=400, pool_size=48, B=128, T=16,
     srv.run(make_requests(32, rate, seed + 1), keep_results=False)  # warm-up
     # Heavier offered loads (same request mix) show throughput under queueing.
     loads = []
-    for k, lam in enumerate((rate * 10, rate * 50)):
+    # The last load runs against a 2 GiB budget, so that admission binds
+    # (requests wait for budget to free up; none needs more than 2 GiB).
+    small = serve.Server(block_size=B, threads=T, budget=2 << 30, streams=3)
+    for k, (lam, sv_) in enumerate(((rate * 10, srv), (rate * 50, srv), (rate * 50, small))):
         rq = make_requests(count, lam, seed + 10 + k)
-        srv.batches = 0
-        srv.peak = 0
+        sv_.batches = 0
+        sv_.peak = 0
         t0 = time.perf_counter()
-        sv = srv.run(rq, keep_results=False)
+        sv = sv_.run(rq, keep_results=False)
         w = time.perf_counter() - t0
         rp = serve.report(sv, w)
-        loads.append({"rate_per_s": lam, "wall_seconds": round(w, 4), "batches": srv.batches,
+        loads.append({"rate_per_s": lam, "budget_GiB": sv_.budget / (1 << 30),
+                      "wall_seconds": round(w, 4), "batches": sv_.batches,
+                      "peak_admitted_bytes": int(sv_.peak), "refused": rp["refused"],
                       "compress_latency_ms": {"p50": rp["compress"]["latency_ms_p50"],
                                               "p99": rp["compress"]["latency_ms_p99"]},
                       "decompress_latency_ms": {"p50": rp["decompress"]["latency_ms_p50"],
                                                 "p99": rp["decompress"]["latency_ms_p99"]},
                       "aggregate_GBps": rp["aggregate_GBps"]})
         del sv, rq
+    del small
     reqs = make_requests(count, rate, seed + 2)
     src = {r.rid: r.result_index for r in reqs}
     srv.batches = 0
