@@ -299,6 +299,7 @@ __device__ uint64_t lookback_sum_wg(uint64_t *st, uint64_t tile, uint64_t first,
 // Tile status: [flag 2][inclusive prefix 40][tile bytes 22]; group status:
 // [flag 2][value 62].
 constexpr uint32_t kLbGroup = 64;       // tiles per look-back group
+constexpr int kLbRawWaves = 2;          // look-back waves that read tile statuses
 constexpr uint32_t kGroupStride = 16;   // u64 words between group statuses
 constexpr uint32_t kAggBits = 22;
 constexpr uint64_t kAggMask = (1ull << kAggBits) - 1;
@@ -355,18 +356,19 @@ __device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile
     if (tile != 0) {
         uint64_t v;
         bool incl;
-        if (wv < 2) {
-            // The earlier tiles of this group and of the one before, nearest
-            // first (the previous group's total comes one round trip after
-            // its last tile's byte count, and that tile may have started
-            // just before this one).
-            const bool in = (uint32_t)tid < j + (g ? kLbGroup : 0u);
+        if (wv < kLbRawWaves) {
+            // The earlier tiles of this group and of the kLbRawWaves - 1
+            // groups before, nearest first (a group's total comes one round
+            // trip after its last tile's byte count, and after the slowest
+            // of its tiles).
+            const uint32_t back = (uint32_t)min<uint64_t>(g, kLbRawWaves - 1) * kLbGroup;
+            const bool in = (uint32_t)tid < j + back;
             const uint64_t w = in ? wait_status(&st[tile - 1 - tid], ctl) : 0;
             incl = in && (w >> 62) == 2;
             v = incl ? ((w >> kAggBits) & kInclMask) : (w & kAggMask);
         } else {
-            // Groups g - 2 .. g - 129, nearest first.
-            const int64_t gi = (int64_t)g - 2 - (int64_t)(tid - 128);
+            // The groups before those, nearest first.
+            const int64_t gi = (int64_t)g - kLbRawWaves - (int64_t)(tid - 64 * kLbRawWaves);
             const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
             incl = (w >> 62) == 2;
             v = w & kValMask;
@@ -385,8 +387,8 @@ __device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile
             }
         }
         // Further back, every thread reads a group (only when no group
-        // among the 128 before has its inclusive prefix yet).
-        int64_t hi = (int64_t)g - 2 - (kWG - 128);
+        // read above has its inclusive prefix yet).
+        int64_t hi = (int64_t)g - kLbRawWaves - (kWG - 64 * kLbRawWaves);
         while (!done) {  // uniform: every thread combines the same slots
             const int64_t gi = hi - tid;
             const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
