@@ -1096,6 +1096,7 @@ __device__ __forceinline__ void compress_simple_waves(
         run += t;
     }
     if (tid < 64) lookback_publish_c(status, gst, tile, run, ctl);
+    __builtin_amdgcn_s_setprio(0);
     uint8_t *reg = ob + (uint32_t)wv * kWaveCap;
     nfit = 0;
     fitbytes = 0;
@@ -1241,6 +1242,7 @@ __device__ __forceinline__ void compress_units_fast(
     }
     run = acc;
     if (tid < 64) lookback_publish_c(status, gst, tile, acc, ctl);
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         if (r * K + d >= nu) continue;
@@ -1347,6 +1349,12 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
                               uint64_t *__restrict__ idx_pos, uint32_t *__restrict__ idx_carry,
                               uint64_t tile, CompressLds &L)
 {
+    // Until the tile's byte count is published its waves issue first: every
+    // later tile's look-back waits for the slowest recent predecessor to
+    // publish, so the work before a publish is on everyone's critical path,
+    // the work after it only on the tile's own (k_compress 214 -> 204 us on
+    // a 512^3 field).
+    __builtin_amdgcn_s_setprio(2);
     const int tid = threadIdx.x, lane = __lane_id();
     const V eb = d_eb ? *d_eb : eb_host;
     // 1 / eb in the value type: FP32 division for float (SZp/src/szp_float.cc:185),
@@ -1434,6 +1442,7 @@ __device__ void compress_tile(const V *__restrict__ x, const V *__restrict__ d_e
     const uint32_t err0 = (EARLY_ERR && tid == 0) ? ld_ctl(&ctl[1]) : 0u;
     {
         if (!(simple || units) && tid < 64) lookback_publish_c(status, gst, tile, run, ctl);
+        __builtin_amdgcn_s_setprio(0);
         const uint64_t p = lookback_compress(status, gst, tile, run, ctl, L.s_lb);
         if (tid == 0) L.s_prefix = p;
     }
