@@ -1743,6 +1743,7 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
 
     const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
     const uint64_t gt = blockIdx.x;
+    __builtin_amdgcn_s_setprio(2);  // until the tile's block count is published
     const uint64_t *cstart = ctab, *aend = ctab + g.T, *ttab = ctab + 2 * g.T;
     if (gt >= ttab[g.T]) return;
     uint32_t lo = 0, hi = g.T - 1;
@@ -1879,7 +1880,9 @@ __global__ __launch_bounds__(kWG) void k_parse(const uint8_t *__restrict__ st, u
     const uint32_t tile_cnt = (uint32_t)s_prefix;
     __syncthreads();
     {
-        const uint64_t pre_cnt = lookback_sum_wg(cnt_st, gt, first, tile_cnt, ctl, false, s_lb);
+        if (tid < 64) lookback_publish(cnt_st, gt, first, tile_cnt);
+        __builtin_amdgcn_s_setprio(0);
+        const uint64_t pre_cnt = lookback_sum_wg(cnt_st, gt, first, tile_cnt, ctl, true, s_lb);
         if (tid == 0) s_prefix = pre_cnt;
     }
     // Pre ranks: pre bits before each word, within its sub-tile.
@@ -2293,6 +2296,7 @@ __device__ __forceinline__ bool decode_simple_fast(
     } else {
         // The tile holds no chunk head: its aggregate is a plain delta sum.
         if (tid < 64) lookback_publish(status, tile, 0, agg);
+        __builtin_amdgcn_s_setprio(0);
         carry = (uint32_t)lookback_sum_wg<true>(status, tile, 0, agg, ctl, true, s_lb);
     }
     const uint64_t Et = item_elem(g, ct, cj);
@@ -2465,6 +2469,10 @@ __device__ __forceinline__ void decode_tile(const uint8_t *__restrict__ st, uint
     uint32_t &s_carry = L.s_carry;
     int &s_bad = L.s_bad;
 
+    // Foreign streams: raised issue priority until the tile's delta sum is
+    // published, as in the compress kernel (successors' look-backs wait for
+    // it).
+    if (!INDEXED) __builtin_amdgcn_s_setprio(2);
     const int tid = threadIdx.x, lane = __lane_id();
     const V eb = get_eb<V>(st);
     const uint64_t tI0 = tile * (uint64_t)g.K * g.R;
