@@ -2840,7 +2840,8 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
     uint32_t carry = 0;
     bool have = pipe_issue<SPB>(st, len, blkpos, idx_carry, cur, rb, pos, carry);
     __syncthreads();
-    for (uint32_t step = 0; cur.tile != ~0ull; ++step) {
+    uint32_t step = 0;
+    for (; cur.tile != ~0ull; ++step) {
         const PipeTile n2 = pipe_plan(st, g, blkpos, nxt.tile == ~0ull ? ~0ull : nxt.tile + G);
         const uint32_t ccur = carry;
         if (have) {
@@ -2871,8 +2872,11 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
         nxt = n2;
     }
     // The workgroup's general tile, then its deferred ones (one call site,
-    // so that decode_tile is inlined).
+    // so that decode_tile is inlined).  Only the list's words that the loop
+    // could have marked are read: scanning all of them, one dependent LDS
+    // read each, took ~14 us at the end of every workgroup.
     bool gen = blockIdx.x < ord.n;
+    const uint32_t nw = (step + 31) / 32;
     uint32_t w = 0, bits = s_defer[0];
     while (true) {
         uint64_t t;
@@ -2880,7 +2884,7 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
             t = ord.t[blockIdx.x];
             gen = false;
         } else {
-            while (!bits && ++w < kPipeMaxSteps / 32) bits = s_defer[w];
+            while (!bits && ++w < nw) bits = s_defer[w];
             if (!bits) break;
             const uint32_t k = (uint32_t)__builtin_ctz(bits);
             bits &= bits - 1;
