@@ -2841,7 +2841,18 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
     bool have = pipe_issue<SPB>(st, len, blkpos, idx_carry, cur, rb, pos, carry);
     __syncthreads();
     uint32_t step = 0;
+    const uint32_t mine = blockIdx.x < g.ntiles ? (uint32_t)((g.ntiles - blockIdx.x + G - 1) / G) : 0u;
+    const uint32_t quarter = (mine + 3) / 4;
     for (; cur.tile != ~0ull; ++step) {
+        // Workgroups that are behind issue first: priority 3 .. 0 by the
+        // quarter of this workgroup's tiles still to go.
+        {
+            const uint32_t pr = quarter ? min(3u, (mine - step) / quarter) : 0u;
+            if (pr == 3) __builtin_amdgcn_s_setprio(3);
+            else if (pr == 2) __builtin_amdgcn_s_setprio(2);
+            else if (pr == 1) __builtin_amdgcn_s_setprio(1);
+            else __builtin_amdgcn_s_setprio(0);
+        }
         const PipeTile n2 = pipe_plan(st, g, blkpos, nxt.tile == ~0ull ? ~0ull : nxt.tile + G);
         const uint32_t ccur = carry;
         if (have) {
