@@ -251,24 +251,42 @@ inline TileOrder general_tiles(const Geom &g, bool with_last = false)
     return o;
 }
 
-// The tile that workgroup b runs.
 // ---- the pipelined decode with the side index ----
-// Workgroup w of a grid of G decodes tiles w, w + G, ...; it passes over at
-// most kPipeMaxSteps of them (its list of deferred tiles has that many bits).
-constexpr uint32_t kPipeMaxSteps = 4096;
+// A workgroup takes at most kPipeMaxSteps tiles (its LDS list of deferred
+// tiles has that many entries).  Its tiles come from one of kPipeClasses
+// ticket counters, ctl[kPipeCtr0 + kPipeCtrStride * c] (inside the 256-byte
+// control block the caller's workspace starts with).
+constexpr uint32_t kPipeMaxSteps = 2048;
+constexpr uint32_t kPipeClasses = 8;
+constexpr uint32_t kPipeCtr0 = 8, kPipeCtrStride = 7;
+static_assert(kPipeCtr0 + kPipeCtrStride * (kPipeClasses - 1) < 64, "counters inside the control block");
+
+// True when grid G lets the classes' workgroups, taking kPipeMaxSteps tiles
+// each, take every tile: floor(G / C) * kPipeMaxSteps >= ceil(ntiles / C)
+// for C = min(G, kPipeClasses) classes.
+inline bool pipe_grid_covers(const Geom &g, uint64_t G)
+{
+    if (G == 0) return g.ntiles == 0;
+    const uint64_t C = G < kPipeClasses ? G : kPipeClasses;
+    return (G / C) * kPipeMaxSteps >= (g.ntiles + C - 1) / C;
+}
 
 // Grid of the pipelined decode: the resident workgroups, at least one per
-// general tile (each decodes one), and enough that no workgroup passes over
-// more than kPipeMaxSteps tiles; never more than the tiles.
+// general tile (each decodes one), never more than the tiles unless the
+// tickets need more workgroups (pipe_grid_covers).
 inline uint64_t pipe_grid_size(const Geom &g, const TileOrder &ord, uint32_t resident)
 {
     uint64_t G = g.ntiles < resident ? g.ntiles : resident;
-    const uint64_t need = (g.ntiles + kPipeMaxSteps - 1) / kPipeMaxSteps;
-    if (G < need) G = need;
     if (G < ord.n) G = ord.n;
+    if (!pipe_grid_covers(g, G)) {
+        const uint64_t per_class = (g.ntiles + kPipeClasses - 1) / kPipeClasses;
+        const uint64_t need = kPipeClasses * ((per_class + kPipeMaxSteps - 1) / kPipeMaxSteps);
+        if (G < need) G = need;
+    }
     return G;
 }
 
+// The tile that workgroup b runs.
 SZP_HD uint64_t tile_of_block(const TileOrder &o, uint64_t b)
 {
     if (o.n == 0) return b;

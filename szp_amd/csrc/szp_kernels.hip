@@ -2812,11 +2812,23 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
     }
 }
 
-// Tiles the pipeline passes over are decoded by decode_tile after it ends,
-// with no pipeline state live: the general tiles (ord: chunk heads, short
-// last blocks and the field's last tile, one per workgroup) and the simple
-// tiles whose bytes exceed kPipeBytes (each workgroup marks its own in an
-// LDS bit list).
+// Tiles are handed out by class: workgroup b is in class c = b % C (C =
+// min(G, kPipeClasses); the dispatcher places workgroups on the XCDs round
+// robin, so a class is one XCD's workgroups) and its class's tiles are
+// c, c + C, c + 2C, ...  Its first three tickets are fixed (its rank r = b / C
+// in the class, then r + n, r + 2n for the class's n workgroups: tile b, then
+// b + G, b + 2G when C divides G); every later one comes from the class's
+// counter, so that a workgroup the CU's arbiter serves late takes fewer
+// tiles instead of finishing last.  The ticket of the tile three steps ahead
+// is requested at each step and read at the next, behind the wait for that
+// step's stream bytes.  A workgroup takes at most kPipeMaxSteps tickets; the
+// grid is sized so that a class's workgroups can take all of its tiles that
+// way (pipe_grid_size).
+// Tiles the pipeline passes over are decoded by decode_tile: the general
+// tiles (ord: chunk heads, short last blocks and the field's last tile, one
+// per workgroup) first, so that the tickets even out their cost, and after
+// the loop the simple tiles whose bytes exceed kPipeBytes (each workgroup
+// lists its own in LDS).
 // At most 128 VGPRs: four workgroups per CU, each with one tile in flight
 // and one being unpacked.
 template <typename V, int SPB>
@@ -2826,34 +2838,34 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
                                                      V *__restrict__ out, uint32_t *__restrict__ ctl,
                                                      TileOrder ord)
 {
-    constexpr uint32_t K = 256 / SPB;
     __shared__ DecodeLds L;
-    __shared__ uint32_t s_defer[kPipeMaxSteps / 32];
-    const uint32_t tid = threadIdx.x, d = tid / SPB, o = tid % SPB;
+    __shared__ uint32_t s_defer[kPipeMaxSteps];  // tickets of the deferred tiles
+    __shared__ uint32_t s_tk, s_ndef;
+    const uint32_t tid = threadIdx.x;
     const uint64_t G = gridDim.x;
     const V eb = get_eb<V>(st);
-    for (uint32_t k = tid; k < kPipeMaxSteps / 32; k += kWG) s_defer[k] = 0;
-    PipeTile cur = pipe_plan(st, g, blkpos, blockIdx.x);
-    PipeTile nxt = pipe_plan(st, g, blkpos, blockIdx.x + G);
+    const uint32_t C = (uint32_t)min<uint64_t>(G, kPipeClasses);
+    const uint32_t c = blockIdx.x % C, r = blockIdx.x / C;
+    const uint32_t nper = (uint32_t)((G - c + C - 1) / C);
+    const uint64_t ctiles = c < g.ntiles ? (g.ntiles - c + C - 1) / C : 0;
+    uint32_t *const ctr = ctl + kPipeCtr0 + kPipeCtrStride * c;
+    auto tile_of = [&](uint32_t t) -> uint64_t { return t < ctiles ? c + (uint64_t)C * t : ~0ull; };
+    if (blockIdx.x < ord.n) {
+        decode_tile<V, true, SPB>(st, len, g, blkpos, idx_carry, out, nullptr, ctl, ord.t[blockIdx.x], L);
+        __syncthreads();
+    }
+    uint32_t kcur = r, knxt = r + nper;
+    uint32_t tk = r + 2 * nper;  // thread 0: the ticket s_tk gets at the next step
+    uint32_t ndef = 0;           // thread 0: deferred tiles listed
+    PipeTile cur = pipe_plan(st, g, blkpos, tile_of(kcur));
+    PipeTile nxt = pipe_plan(st, g, blkpos, tile_of(knxt));
     uint4 rb[kPipeVec];
     uint64_t pos[kMaxRounds];
     uint32_t carry = 0;
     bool have = pipe_issue<SPB>(st, len, blkpos, idx_carry, cur, rb, pos, carry);
-    __syncthreads();
-    uint32_t step = 0;
-    const uint32_t mine = blockIdx.x < g.ntiles ? (uint32_t)((g.ntiles - blockIdx.x + G - 1) / G) : 0u;
-    const uint32_t quarter = (mine + 3) / 4;
-    for (; cur.tile != ~0ull; ++step) {
-        // Workgroups that are behind issue first: priority 3 .. 0 by the
-        // quarter of this workgroup's tiles still to go.
-        {
-            const uint32_t pr = quarter ? min(3u, (mine - step) / quarter) : 0u;
-            if (pr == 3) __builtin_amdgcn_s_setprio(3);
-            else if (pr == 2) __builtin_amdgcn_s_setprio(2);
-            else if (pr == 1) __builtin_amdgcn_s_setprio(1);
-            else __builtin_amdgcn_s_setprio(0);
-        }
-        const PipeTile n2 = pipe_plan(st, g, blkpos, nxt.tile == ~0ull ? ~0ull : nxt.tile + G);
+    for (uint32_t step = 0; cur.tile != ~0ull; ++step) {
+        constexpr uint32_t K = 256 / SPB;
+        const uint32_t d = tid / SPB, o = tid % SPB;
         const uint32_t ccur = carry;
         if (have) {
             // This tile's bytes and slot positions go to LDS ...
@@ -2865,43 +2877,38 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
             }
             if (o == 0) {
 #pragma unroll
-                for (int r = 0; r < kMaxRounds; ++r)
-                    L.s_pos[r * K + d] = (pos[r] >= cur.P0 && pos[r] < cur.P1)
-                                             ? (uint32_t)(pos[r] - a0)
+                for (int q = 0; q < kMaxRounds; ++q)
+                    L.s_pos[q * K + d] = (pos[q] >= cur.P0 && pos[q] < cur.P1)
+                                             ? (uint32_t)(pos[q] - a0)
                                              : ~0u;
             }
-        } else if (cur.simple && tid == 0) {
-            s_defer[step >> 5] |= 1u << (step & 31);
+        }
+        if (tid == 0) {
+            if (!have && cur.simple) s_defer[ndef++] = kcur;
+            // The ticket asked for at the last step (its answer came before
+            // the stream bytes above), and the next one: none once a ticket
+            // lies past the class's tiles (every later one does too).
+            s_tk = tk;
+            tk = (tk < ctiles && step + 3 < kPipeMaxSteps) ? 3 * nper + atomicAdd(ctr, 1u) : ~0u;
         }
         __syncthreads();
+        const uint32_t kn2 = __builtin_amdgcn_readfirstlane(s_tk);
+        const PipeTile n2 = pipe_plan(st, g, blkpos, tile_of(kn2));
         // ... the next tile's loads go out, then this tile is unpacked.
         const bool have_n = pipe_issue<SPB>(st, len, blkpos, idx_carry, nxt, rb, pos, carry);
         if (have) pipe_decode<V, SPB>(cur, ccur, eb, out, ctl, L);
-        __syncthreads();  // L is reused by the next tile
+        __syncthreads();  // L and s_tk are reused by the next step
         have = have_n;
         cur = nxt;
         nxt = n2;
+        kcur = knxt;
+        knxt = kn2;
     }
-    // The workgroup's general tile, then its deferred ones (one call site,
-    // so that decode_tile is inlined).  Only the list's words that the loop
-    // could have marked are read: scanning all of them, one dependent LDS
-    // read each, took ~14 us at the end of every workgroup.
-    bool gen = blockIdx.x < ord.n;
-    const uint32_t nw = (step + 31) / 32;
-    uint32_t w = 0, bits = s_defer[0];
-    while (true) {
-        uint64_t t;
-        if (gen) {
-            t = ord.t[blockIdx.x];
-            gen = false;
-        } else {
-            while (!bits && ++w < nw) bits = s_defer[w];
-            if (!bits) break;
-            const uint32_t k = (uint32_t)__builtin_ctz(bits);
-            bits &= bits - 1;
-            t = blockIdx.x + (uint64_t)(32 * w + k) * G;
-        }
-        decode_tile<V, true, SPB>(st, len, g, blkpos, idx_carry, out, nullptr, ctl, t, L);
+    if (tid == 0) s_ndef = ndef;
+    __syncthreads();
+    const uint32_t nd = s_ndef;
+    for (uint32_t k = 0; k < nd; ++k) {
+        decode_tile<V, true, SPB>(st, len, g, blkpos, idx_carry, out, nullptr, ctl, tile_of(s_defer[k]), L);
         __syncthreads();
     }
 }
@@ -3736,7 +3743,10 @@ static int launch_decompress_indexed_t(int mode, const uint8_t *d_stream, uint64
     if (idx_bytes < index_bytes(n, B, T, mode) || work_bytes < 256) return kCap;
     if (len < g.body) return kArg;
     uint32_t *ctl = reinterpret_cast<uint32_t *>(d_work);
-    SZP_LAUNCH("k_clear", k_clear, dim3(1), dim3(kWG), 0, stream, ctl, nullptr, 0);
+    // The whole 256-byte control block: error word and the pipelined
+    // decode's ticket counters (words 8..63, cleared as 28 status words).
+    SZP_LAUNCH("k_clear", k_clear, dim3(1), dim3(kWG), 0, stream, ctl,
+               reinterpret_cast<uint64_t *>(ctl + 8), 28);
     if (n == 0) return hipGetLastError() == hipSuccess ? kOk : kHip;
     if (mode == 2 && B == 1) {
         // As in the parse path: the reference's decoding of units of one value.
@@ -3757,7 +3767,9 @@ static int launch_decompress_indexed_t(int mode, const uint8_t *d_stream, uint64
         switch (spb) {
 #define SZP_PIPE_CASE(S)                                                                         \
     case S: {                                                                                    \
-        const dim3 grid((uint32_t)pipe_grid_size(g, ord, pipe_grid<V, S>()));                    \
+        const uint64_t G = pipe_grid_size(g, ord, pipe_grid<V, S>());                              \
+        if (G > 0xffffffffu || !pipe_grid_covers(g, G)) return kArg;                             \
+        const dim3 grid((uint32_t)G);                                                            \
         SZP_LAUNCH("k_decode_indexed", (k_decode_pipe<V, S>), grid, dim3(kWG), 0, stream,        \
                    d_stream, len, g, ipos, icarry, d_out, ctl, ord);                             \
         break;                                                                                   \

@@ -9,6 +9,7 @@
 #include <stdint.h>
 #include <vector>
 #include <random>
+#include <algorithm>
 
 #include "szp_geom.h"
 
@@ -185,17 +186,61 @@ static void check_order(const Geom &g, bool with_last)
     }
     // Without a list, every tile takes blockIdx order.
     if (!in_use) CHECK(tile_of_block(o, 5) == 5, "identity order");
-    // The pipelined decode's grid: workgroup w takes tiles w, w + G, ... and
-    // the general tile o.t[w] for w < o.n.
+    // The pipelined decode's grid: workgroup w takes tiles of its class by
+    // ticket (check_pipe_tickets) and the general tile o.t[w] for w < o.n.
     if (with_last && in_use && g.ntiles) {
         for (uint32_t resident : {1u, 3u, 1024u, 1u << 20}) {
             const uint64_t G = pipe_grid_size(g, o, resident);
             CHECK(G >= 1 && G <= g.ntiles, "pipe grid %llu for %llu tiles", (unsigned long long)G,
                   (unsigned long long)g.ntiles);
             CHECK(G >= o.n, "pipe grid below the general tiles");
-            CHECK((g.ntiles + G - 1) / G <= kPipeMaxSteps, "pipe steps exceed the deferral bits");
+            CHECK(pipe_grid_covers(g, G), "pipe grid leaves tiles untaken");
         }
     }
+}
+
+// The pipelined decode's tickets (k_decode_pipe), stepped workgroup by
+// workgroup in a random order: every tile of every class must be decoded
+// exactly once and no workgroup may take more than kPipeMaxSteps tiles.
+static void check_pipe_tickets(uint64_t ntiles, uint64_t G, uint64_t seed)
+{
+    const uint32_t C = (uint32_t)std::min<uint64_t>(G, kPipeClasses);
+    struct Wg { uint32_t kcur, knxt, tk, step; bool done; };
+    std::vector<uint32_t> ctr(C, 0), seen(ntiles, 0);
+    std::vector<Wg> w(G);
+    auto ctiles = [&](uint32_t c) -> uint64_t { return c < ntiles ? (ntiles - c + C - 1) / C : 0; };
+    auto nper = [&](uint32_t c) -> uint32_t { return (uint32_t)((G - c + C - 1) / C); };
+    for (uint64_t b = 0; b < G; ++b) {
+        const uint32_t c = (uint32_t)(b % C), r = (uint32_t)(b / C);
+        w[b] = {r, r + nper(c), r + 2 * nper(c), 0, false};
+    }
+    std::mt19937_64 rng(seed);
+    uint64_t live = G;
+    while (live) {
+        const uint64_t b = rng() % G;
+        Wg &x = w[b];
+        if (x.done) continue;
+        const uint32_t c = (uint32_t)(b % C);
+        if (x.kcur >= ctiles(c)) {
+            x.done = true;
+            --live;
+            continue;
+        }
+        ++seen[c + (uint64_t)C * x.kcur];
+        const uint32_t s_tk = x.tk;
+        x.tk = (x.tk < ctiles(c) && x.step + 3 < kPipeMaxSteps) ? 3 * nper(c) + ctr[c]++ : ~0u;
+        x.kcur = x.knxt;
+        x.knxt = s_tk;
+        ++x.step;
+        CHECK(x.step <= kPipeMaxSteps, "workgroup took more than kPipeMaxSteps tiles");
+    }
+    bool all = true;
+    for (uint64_t t = 0; t < ntiles; ++t) {
+        CHECK(seen[t] <= 1, "tile %llu decoded %u times", (unsigned long long)t, seen[t]);
+        all = all && seen[t] == 1;
+    }
+    CHECK(all, "tickets left tiles undecoded (ntiles %llu, G %llu)",
+                            (unsigned long long)ntiles, (unsigned long long)G);
 }
 
 static void check_geometry(uint64_t n, int B, int T, bool full, uint32_t mode = 1)
@@ -246,6 +291,24 @@ int main()
     check_geometry(4194307ull, 128, 8, false);
     check_geometry(134217728ull, 128, 16, false, 2);
     check_geometry(4294967296ull, 64, 16, false, 2);
+    // Ticket coverage: small and ragged grids, a grid below the classes, the
+    // 512^3 field's 16385 tiles on 1024 workgroups, 16384 tiles on 8, and grids that
+    // cover exactly (every workgroup takes kPipeMaxSteps tiles).
+    const uint64_t cases[][2] = {{1, 1},       {5, 3},    {9, 8},    {100, 13},   {1000, 7},
+                                 {16385, 1024}, {16384, 8}, {16385, 17}, {32768, 16}, {32769, 24},
+                                 {65536, 32}};
+    for (const auto &cs : cases) {
+        Geom g;
+        g.ntiles = cs[0];
+        CHECK(pipe_grid_covers(g, cs[1]), "case grid %llu does not cover %llu tiles",
+              (unsigned long long)cs[1], (unsigned long long)cs[0]);
+        for (uint64_t seed = 1; seed <= 3; ++seed) check_pipe_tickets(cs[0], cs[1], seed);
+    }
+    {
+        Geom g;
+        g.ntiles = 32769;
+        CHECK(!pipe_grid_covers(g, 16), "a grid one tile short must be rejected");
+    }
     printf("geom_check: %llu checks, %d failures\n", (unsigned long long)g_checks, g_fail);
     return g_fail ? 1 : 0;
 }
