@@ -2815,13 +2815,13 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
 // Tiles are handed out by class: workgroup b is in class c = b % C (C =
 // min(G, kPipeClasses); the dispatcher places workgroups on the XCDs round
 // robin, so a class is one XCD's workgroups) and its class's tiles are
-// c, c + C, c + 2C, ...  Its first three tickets are fixed (its rank r = b / C
-// in the class, then r + n, r + 2n for the class's n workgroups: tile b, then
-// b + G, b + 2G when C divides G); every later one comes from the class's
-// counter, so that a workgroup the CU's arbiter serves late takes fewer
-// tiles instead of finishing last.  The ticket of the tile three steps ahead
-// is requested at each step and read at the next, behind the wait for that
-// step's stream bytes.  A workgroup takes at most kPipeMaxSteps tickets; the
+// c, c + C, c + 2C, ...  Its first four tickets are fixed (its rank r = b / C
+// in the class, then r + n, r + 2n, r + 3n for the class's n workgroups: tile
+// b, then b + G, ... when C divides G); every later one comes from the
+// class's counter, so that a workgroup the CU's arbiter serves late takes
+// fewer tiles instead of finishing last.  The ticket of the tile four steps
+// ahead is requested at each step and read at the next, behind the wait for
+// that step's stream bytes.  A workgroup takes at most kPipeMaxSteps tickets; the
 // grid is sized so that a class's workgroups can take all of its tiles that
 // way (pipe_grid_size).
 // Tiles the pipeline passes over are decoded by decode_tile: the general
@@ -2854,9 +2854,11 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
         decode_tile<V, true, SPB>(st, len, g, blkpos, idx_carry, out, nullptr, ctl, ord.t[blockIdx.x], L);
         __syncthreads();
     }
-    uint32_t kcur = r, knxt = r + nper;
-    uint32_t tk = r + 2 * nper;  // thread 0: the ticket s_tk gets at the next step
-    uint32_t ndef = 0;           // thread 0: deferred tiles listed
+    // Tickets of the current tile and the next two; thread 0 holds `got`,
+    // the counter's answer to the last step's request when `asked`.
+    uint32_t kcur = r, knxt = r + nper, kn2 = r + 2 * nper, got = 0;
+    bool asked = false;
+    if (tid == 0) s_ndef = 0;  // thread 0 alone lists deferred tiles
     PipeTile cur = pipe_plan(st, g, blkpos, tile_of(kcur));
     PipeTile nxt = pipe_plan(st, g, blkpos, tile_of(knxt));
     uint4 rb[kPipeVec];
@@ -2866,6 +2868,7 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
     for (uint32_t step = 0; cur.tile != ~0ull; ++step) {
         constexpr uint32_t K = 256 / SPB;
         const uint32_t d = tid / SPB, o = tid % SPB;
+        const PipeTile n2 = pipe_plan(st, g, blkpos, tile_of(kn2));
         const uint32_t ccur = carry;
         if (have) {
             // This tile's bytes and slot positions go to LDS ...
@@ -2884,16 +2887,27 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
             }
         }
         if (tid == 0) {
-            if (!have && cur.simple) s_defer[ndef++] = kcur;
-            // The ticket asked for at the last step (its answer came before
-            // the stream bytes above), and the next one: none once a ticket
-            // lies past the class's tiles (every later one does too).
-            s_tk = tk;
-            tk = (tk < ctiles && step + 3 < kPipeMaxSteps) ? 3 * nper + atomicAdd(ctr, 1u) : ~0u;
+            if (!have && cur.simple) s_defer[s_ndef++] = kcur;
+            // The ticket of the tile three steps ahead (the counter answered
+            // the last step's request before the stream bytes above came),
+            // and the request for the one after it: none once a ticket lies
+            // past the class's tiles (every later one does too).  The answer
+            // is used only at the next step, so the request is not waited on:
+            // the address is offset by a register zero the compiler cannot
+            // see through, which keeps its wave-uniform atomic rewrite (one
+            // atomic for the wave, its answer read into a scalar at once)
+            // away.
+            const uint32_t k3 = asked ? 4 * nper + got : step == 0 ? r + 3 * nper : ~0u;
+            s_tk = k3;
+            asked = k3 < ctiles && step + 4 < kPipeMaxSteps;
+            if (asked) {
+                uint32_t z;
+                asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+                got = atomicAdd(ctr + z, 1u);
+            }
         }
         __syncthreads();
-        const uint32_t kn2 = __builtin_amdgcn_readfirstlane(s_tk);
-        const PipeTile n2 = pipe_plan(st, g, blkpos, tile_of(kn2));
+        const uint32_t kn3 = __builtin_amdgcn_readfirstlane(s_tk);
         // ... the next tile's loads go out, then this tile is unpacked.
         const bool have_n = pipe_issue<SPB>(st, len, blkpos, idx_carry, nxt, rb, pos, carry);
         if (have) pipe_decode<V, SPB>(cur, ccur, eb, out, ctl, L);
@@ -2903,8 +2917,8 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
         nxt = n2;
         kcur = knxt;
         knxt = kn2;
+        kn2 = kn3;
     }
-    if (tid == 0) s_ndef = ndef;
     __syncthreads();
     const uint32_t nd = s_ndef;
     for (uint32_t k = 0; k < nd; ++k) {
