@@ -2815,16 +2815,15 @@ __device__ __forceinline__ void pipe_decode(const PipeTile &p, uint32_t carry, V
 // Tiles are handed out by class: workgroup b is in class c = b % C (C =
 // min(G, kPipeClasses); the dispatcher places workgroups on the XCDs round
 // robin, so a class is one XCD's workgroups) and its class's tiles are
-// c, c + C, c + 2C, ...  Its first three tickets are fixed (its rank r = b / C
-// in the class, then r + n, r + 2n for the class's n workgroups: tile b, then
-// b + G, b + 2G when C divides G); every later one comes from the class's
-// counter, so that a workgroup the CU's arbiter serves late takes fewer
-// tiles instead of finishing last.  The ticket of the tile three steps ahead
-// is requested at each step and read at the next, behind the wait for that
-// step's stream bytes.  A workgroup holds at most four unfinished tickets
-// (the current tile, the next, the ticket after and the one asked for):
-// when a class's counter runs dry its workgroups end that many of their own
-// steps later, so fewer is a shorter tail.  A workgroup takes at most kPipeMaxSteps tickets; the
+// c, c + C, c + 2C, ...  Its first two tickets are fixed (its rank r = b / C
+// in the class, then r + n for the class's n workgroups: tile b, then b + G
+// when C divides G); every later one comes from the class's counter, so that
+// a workgroup the CU's arbiter serves late takes fewer tiles instead of
+// finishing last.  The ticket of the tile two steps ahead is asked for as a
+// step's stream bytes are staged and read at the step's end.  A workgroup
+// holds at most three unfinished tickets (the current tile, the next and the
+// one asked for): when a class's counter runs dry its workgroups end that
+// many of their own steps later, so fewer is a shorter tail.  A workgroup takes at most kPipeMaxSteps tickets; the
 // grid is sized so that a class's workgroups can take all of its tiles that
 // way (pipe_grid_size).
 // Tiles the pipeline passes over are decoded by decode_tile: the general
@@ -2858,7 +2857,7 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
         __syncthreads();
     }
     // Tickets of the current tile and of the one whose loads go out at this
-    // step; thread 0 holds `got`, the counter's answer to the last step's
+    // step; thread 0 holds `got`, the counter's answer to this step's
     // request when `asked`.
     uint32_t kcur = r, knxt = r + nper, got = 0;
     bool asked = false;
@@ -2893,18 +2892,13 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
         }
         if (tid == 0) {
             if (!have && cur.simple) s_defer[s_ndef++] = kcur;
-            // The ticket of the tile after next (the counter answered the
-            // last step's request before the stream bytes above came), and
-            // the request for the one after it: none once a ticket lies past
-            // the class's tiles (every later one does too).  The answer is
-            // used only at the next step, so the request is not waited on:
-            // the address is offset by a register zero the compiler cannot
-            // see through, which keeps its wave-uniform atomic rewrite (one
-            // atomic for the wave, its answer read into a scalar at once)
-            // away.
-            const uint32_t k2 = asked ? 3 * nper + got : step == 0 ? r + 2 * nper : ~0u;
-            s_tk = k2;
-            asked = k2 < ctiles && step + 3 < kPipeMaxSteps;
+            // The ticket of the tile after next, read at the end of this
+            // step: none once a ticket lies past the class's tiles (every
+            // later one does too).  The address is offset by a register zero
+            // the compiler cannot see through, which keeps its wave-uniform
+            // atomic rewrite (one atomic for the wave, its answer read into a
+            // scalar at once) away.
+            asked = knxt < ctiles && step + 2 < kPipeMaxSteps;
             if (asked) {
                 uint32_t z;
                 asm volatile("v_mov_b32 %0, 0" : "=v"(z));
@@ -2912,15 +2906,15 @@ __global__ __launch_bounds__(kWG, 4) void k_decode_pipe(const uint8_t *__restric
             }
         }
         __syncthreads();
-        const uint32_t kn2 = __builtin_amdgcn_readfirstlane(s_tk);
         // ... the next tile's loads go out, then this tile is unpacked.
         const bool have_n = pipe_issue<SPB>(st, len, blkpos, idx_carry, nxt, rb, pos, carry);
         if (have) pipe_decode<V, SPB>(cur, ccur, eb, out, ctl, L);
+        if (tid == 0) s_tk = asked ? 2 * nper + got : ~0u;
         __syncthreads();  // L and s_tk are reused by the next step
         have = have_n;
         cur = nxt;
         kcur = knxt;
-        knxt = kn2;
+        knxt = __builtin_amdgcn_readfirstlane(s_tk);
     }
     __syncthreads();
     const uint32_t nd = s_ndef;

@@ -205,14 +205,14 @@ static void check_order(const Geom &g, bool with_last)
 static void check_pipe_tickets(uint64_t ntiles, uint64_t G, uint64_t seed)
 {
     const uint32_t C = (uint32_t)std::min<uint64_t>(G, kPipeClasses);
-    struct Wg { uint32_t kcur, knxt, got, step; bool asked, done; };
+    struct Wg { uint32_t kcur, knxt, step; bool done; };
     std::vector<uint32_t> ctr(C, 0), seen(ntiles, 0);
     std::vector<Wg> w(G);
     auto ctiles = [&](uint32_t c) -> uint64_t { return c < ntiles ? (ntiles - c + C - 1) / C : 0; };
     auto nper = [&](uint32_t c) -> uint32_t { return (uint32_t)((G - c + C - 1) / C); };
     for (uint64_t b = 0; b < G; ++b) {
         const uint32_t c = (uint32_t)(b % C), r = (uint32_t)(b / C), n = nper(c);
-        w[b] = {r, r + n, 0, 0, false, false};
+        w[b] = {r, r + n, 0, false};
     }
     std::mt19937_64 rng(seed);
     uint64_t live = G;
@@ -220,16 +220,15 @@ static void check_pipe_tickets(uint64_t ntiles, uint64_t G, uint64_t seed)
         const uint64_t b = rng() % G;
         Wg &x = w[b];
         if (x.done) continue;
-        const uint32_t c = (uint32_t)(b % C), r = (uint32_t)(b / C);
+        const uint32_t c = (uint32_t)(b % C);
         if (x.kcur >= ctiles(c)) {
             x.done = true;
             --live;
             continue;
         }
         ++seen[c + (uint64_t)C * x.kcur];
-        const uint32_t k2 = x.asked ? 3 * nper(c) + x.got : x.step == 0 ? r + 2 * nper(c) : ~0u;
-        x.asked = k2 < ctiles(c) && x.step + 3 < kPipeMaxSteps;
-        if (x.asked) x.got = ctr[c]++;
+        const bool asked = x.knxt < ctiles(c) && x.step + 2 < kPipeMaxSteps;
+        const uint32_t k2 = asked ? 2 * nper(c) + ctr[c]++ : ~0u;
         x.kcur = x.knxt;
         x.knxt = k2;
         ++x.step;
