@@ -67,6 +67,23 @@ def make_input(kind, n, seed):
         return x
     if kind == "posinf":
         return np.full(n, np.inf, dtype=np.float32)
+    if kind == "nearint":
+        # Values whose products x * fl32(1 / 0.001) lie within a few ulps of
+        # an integer k, |k| from 1 to 2^31, on a walk: the quantizer's
+        # truncation of the exact product is tested where a float product
+        # would round onto the integer.
+        x = _walk(rng, n)
+        inv = np.float32(1.0) / np.float32(0.001)
+        m = n // 2
+        kk = np.floor(2.0 ** (31.0 * rng.integers(0, 1 << 20, size=m) / (1 << 20)))
+        kk *= np.where(rng.integers(0, 2, size=m) == 1, -1.0, 1.0)
+        near = (kk / np.float64(inv)).astype(np.float32)
+        step = rng.integers(-3, 4, size=m)
+        for j in range(3):
+            near = np.where(step > j, np.nextafter(near, np.float32(np.inf)), near)
+            near = np.where(step < -j, np.nextafter(near, np.float32(-np.inf)), near)
+        x[rng.permutation(n)[:m]] = near
+        return x
     if kind == "nanfirst":
         x = _walk(rng, n)
         x[0] = np.float32(np.nan)

@@ -87,6 +87,8 @@ CASES = [
     ("constant", 100000, 32, 4, 1, 0.0, 1e-3),
     ("constant", 100000, 128, 4, 0, 0.5, 0.0),
     ("specials", 200000, 32, 4, 0, 0.5, 0.0),
+    ("nearint", 200000, 128, 4, 0, 0.001, 0.0),  # products within ulps of integers
+    ("nearint", 65536, 32, 2, 0, 0.001, 0.0),
     ("sparsewide", 200000, 64, 2, 0, 1.0, 0.0),
     ("sparsewide", 200000, 16, 5, 0, 0.25, 0.0),
     ("tiny", 50000, 32, 2, 0, 5e-39, 0.0),
