@@ -2672,12 +2672,13 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
 // ---------------------------------------------------------------------------
 //
 // With the side index a tile needs nothing from any other tile, so one
-// workgroup can decode tiles w, w + G, w + 2G, ... (G = gridDim.x) in a
-// pipeline: while it unpacks tile c from LDS, the stream bytes, block
-// positions and carry of tile c + G are already in flight into registers,
-// and the byte range of tile c + 2G is being read.  A tile's two dependent
-// memory latencies (positions, then bytes) thus overlap the previous tile's
-// work instead of adding to it.  Simple tiles (full blocks of one chunk, B/8 a
+// workgroup can decode a stream of tiles (taken by ticket, below) in a
+// pipeline: while it unpacks the current tile from LDS, the stream bytes,
+// block positions and carry of the next one are in flight into registers.
+// The next tile's byte range is read at the top of the step and its bytes
+// after the step's first barrier, so a tile's two dependent memory latencies
+// (positions, then bytes) overlap the current tile's work instead of adding
+// to it.  Simple tiles (full blocks of one chunk, B/8 a
 // power of two up to 32) whose bytes fit kPipeBytes go through the pipeline;
 // every other tile (chunk heads, the field's last tile, barely compressible
 // tiles) is decoded in place by decode_tile.  No tile waits for another, so
@@ -2698,7 +2699,7 @@ struct PipeTile {
 };
 
 // Locates tile `tile` and, for a simple tile, issues the loads of its byte
-// range (used one pipeline step later).
+// range (used after the step's first barrier, by pipe_issue).
 __device__ __forceinline__ PipeTile pipe_plan(const uint8_t *__restrict__ st, const Geom &g,
                                               const uint64_t *__restrict__ blkpos, uint64_t tile)
 {
