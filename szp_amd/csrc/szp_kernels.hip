@@ -416,6 +416,118 @@ __device__ uint64_t lookback_compress(uint64_t *st, uint64_t *gst, uint64_t tile
     return excl;
 }
 
+// ---- foreign-stream decode look-back over tiles and groups of tiles ----
+//
+// The same grouping for the delta sums (mod 2^32) of the look-back decode.
+// Its status words live at `st` (one per tile, the value in the low 32
+// bits), then a tile's own first word (never overwritten: the group's last
+// tile sums them even after a tile's status has turned inclusive), then the
+// group statuses, kGroupStride words apart.  A tile that holds a chunk head
+// is absolute: its first word is already inclusive, and a group's word is
+// inclusive from the group's last absolute tile on.  Only fastMode-1 simple
+// tiles read group words, so fastMode 2 publishes none.
+__device__ __forceinline__ uint64_t *dec_own(uint64_t *st, const Geom &g) { return st + g.ntiles; }
+__device__ __forceinline__ uint64_t *dec_groups(uint64_t *st, const Geom &g) { return st + 2 * g.ntiles; }
+
+// Publishes a decode tile's first word (kFlagAgg | sum, or kFlagIncl | value
+// for an absolute tile); wave 0 calls it (all 64 lanes).  A group's last
+// tile then publishes the group's word from the group's 64 first words.
+__device__ void lookback_publish_d(uint64_t *st, const Geom &g, uint64_t tile, uint64_t word,
+                                   uint32_t *ctl)
+{
+    const int lane = __lane_id();
+    if (lane == 0) {
+        st_status(&dec_own(st, g)[tile], word);
+        st_status(&st[tile], word);
+    }
+    if (g.mode != 1 || tile % kLbGroup != kLbGroup - 1) return;
+    const uint64_t w = lane == 63 ? word : wait_status(&dec_own(st, g)[tile - 63 + lane], ctl);
+    const uint64_t inc = __ballot((w >> 62) == 2);
+    const int h = inc ? 63 - __builtin_clzll(inc) : 0;  // the last absolute tile
+    const uint32_t sum = wave_sum_u32(lane >= h ? (uint32_t)w : 0u);
+    if (lane == 0)
+        st_status(&dec_groups(st, g)[(tile / kLbGroup) * kGroupStride], (inc ? kFlagIncl : kFlagAgg) | sum);
+}
+
+// Publishes a decode tile's inclusive value, and its group's for a group's
+// last tile (lane 0 of the caller).
+__device__ __forceinline__ void lookback_finish_d(uint64_t *st, const Geom &g, uint64_t tile,
+                                                  uint32_t inc)
+{
+    st_status(&st[tile], kFlagIncl | inc);
+    if (g.mode == 1 && tile % kLbGroup == kLbGroup - 1)
+        st_status(&dec_groups(st, g)[(tile / kLbGroup) * kGroupStride], kFlagIncl | inc);
+}
+
+// A tile that stops on an error still publishes (inclusive, value 0) its
+// status, its first word and its group's word, so that no tile waits for it.
+__device__ __forceinline__ void publish_abort_d(uint64_t *st, const Geom &g, uint64_t tile)
+{
+    st_status(&dec_own(st, g)[tile], kFlagIncl);
+    lookback_finish_d(st, g, tile, 0);
+}
+
+// Look-back of a decode simple tile (never absolute, never tile 0) after
+// lookback_publish_d, in the shape of lookback_compress: returns the
+// exclusive prefix mod 2^32 in every thread and publishes the inclusive one.
+__device__ uint32_t lookback_decode(uint64_t *st, const Geom &g, uint64_t tile, uint32_t agg,
+                                    uint32_t *ctl, uint64_t *s_lb)
+{
+    const int tid = threadIdx.x, wv = tid >> 6, lane = __lane_id();
+    uint64_t *gst = dec_groups(st, g);
+    const uint64_t gr = tile / kLbGroup;
+    const uint32_t j = (uint32_t)(tile % kLbGroup);
+    uint64_t v;
+    bool incl;
+    if (wv < kLbRawWaves) {
+        const uint32_t back = (uint32_t)min<uint64_t>(gr, kLbRawWaves - 1) * kLbGroup;
+        const bool in = (uint32_t)tid < j + back;
+        const uint64_t w = in ? wait_status(&st[tile - 1 - tid], ctl) : 0;
+        incl = in && (w >> 62) == 2;
+        v = (uint32_t)w;
+    } else {
+        const int64_t gi = (int64_t)gr - kLbRawWaves - (int64_t)(tid - 64 * kLbRawWaves);
+        const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
+        incl = (w >> 62) == 2;
+        v = (uint32_t)w;
+    }
+    bool found;
+    const uint64_t part = wave_lookback_part(v, incl, found);
+    if (lane == 0) s_lb[wv] = (part << 1) | (found ? 1u : 0u);
+    __syncthreads();
+    uint64_t excl = 0;
+    bool done = false;
+#pragma unroll
+    for (int w = 0; w < kWG / 64; ++w) {
+        const uint64_t e = s_lb[w];
+        if (!done) {
+            excl += e >> 1;
+            done = e & 1;
+        }
+    }
+    int64_t hi = (int64_t)gr - kLbRawWaves - (kWG - 64 * kLbRawWaves);
+    while (!done) {  // uniform
+        const int64_t gi = hi - tid;
+        const uint64_t w = gi >= 0 ? wait_status(&gst[(uint64_t)gi * kGroupStride], ctl) : kFlagIncl;
+        bool fw;
+        const uint64_t pw = wave_lookback_part((uint32_t)w, (w >> 62) == 2, fw);
+        __syncthreads();
+        if (lane == 0) s_lb[4 + wv] = (pw << 1) | (fw ? 1u : 0u);
+        __syncthreads();
+#pragma unroll
+        for (int w2 = 0; w2 < kWG / 64; ++w2) {
+            const uint64_t e = s_lb[4 + w2];
+            if (!done) {
+                excl += e >> 1;
+                done = e & 1;
+            }
+        }
+        hi -= kWG;
+    }
+    if (tid == 0) lookback_finish_d(st, g, tile, (uint32_t)excl + agg);
+    return (uint32_t)excl;
+}
+
 // Copies tot bytes from LDS to out[G0, G0 + tot): 16-byte stores for the
 // aligned middle, byte stores for the ragged ends shared with neighbours.
 __device__ void copy_out_bytes(uint8_t *__restrict__ out, uint64_t G0, uint32_t tot,
@@ -2289,7 +2401,7 @@ __device__ __forceinline__ bool decode_simple_fast(
 #pragma unroll
     for (int k = 0; k < kMaxRounds * 4; ++k) agg += s_wsum[k];
     if (*s_bad) {
-        if (!INDEXED && tid == 0) st_status(&status[tile], kFlagIncl);
+        if (!INDEXED && tid == 0) publish_abort_d(status, g, tile);
         if (tid == 0) raise_error(ctl, kErrStream);
         return true;
     }
@@ -2298,9 +2410,9 @@ __device__ __forceinline__ bool decode_simple_fast(
         carry = icarry;
     } else {
         // The tile holds no chunk head: its aggregate is a plain delta sum.
-        if (tid < 64) lookback_publish(status, tile, 0, agg);
+        if (tid < 64) lookback_publish_d(status, g, tile, kFlagAgg | agg, ctl);
         __builtin_amdgcn_s_setprio(0);
-        carry = (uint32_t)lookback_sum_wg<true>(status, tile, 0, agg, ctl, true, s_lb);
+        carry = lookback_decode(status, g, tile, agg, ctl, s_lb);
     }
     const uint64_t Et = item_elem(g, ct, cj);
     uint32_t base = carry;
@@ -2616,10 +2728,10 @@ __device__ __forceinline__ void decode_tile(const uint8_t *__restrict__ st, uint
         if (s_bad) break;
         // Look-back: a tile whose aggregate contains a head is absolute by itself.
         if (tid < 64) {
-            if (th && lane == 0) st_status(&status[tile], kFlagIncl | tv);
+            const bool lb = tile > 0 && !item_head(g, j0);
+            lookback_publish_d(status, g, tile, ((th || !lb) ? kFlagIncl : kFlagAgg) | tv, ctl);
             uint32_t c = 0;
-            if (tile > 0 && !item_head(g, j0)) {
-                if (!th && lane == 0) st_status(&status[tile], kFlagAgg | tv);
+            if (lb) {
                 int64_t hi = (int64_t)tile - 1;
                 while (true) {
                     const int64_t idx = hi - lane;
@@ -2633,9 +2745,7 @@ __device__ __forceinline__ void decode_tile(const uint8_t *__restrict__ st, uint
                     c += wave_sum_u32((uint32_t)sx);
                     hi -= 64;
                 }
-                if (!th && lane == 0) st_status(&status[tile], kFlagIncl | (uint32_t)(c + tv));
-            } else if (!th && lane == 0) {
-                st_status(&status[tile], kFlagIncl | tv);
+                if (!th && lane == 0) lookback_finish_d(status, g, tile, c + tv);
             }
             if (lane == 0) s_carry = c;
         }
@@ -2644,7 +2754,7 @@ __device__ __forceinline__ void decode_tile(const uint8_t *__restrict__ st, uint
     __syncthreads();
     if (s_bad && tid == 0) {
         // Publish (harmlessly, if already published) so no successor waits.
-        if (!INDEXED) st_status(&status[tile], kFlagIncl);
+        if (!INDEXED) publish_abort_d(status, g, tile);
         raise_error(ctl, kErrStream);
     }
 }
@@ -2661,7 +2771,7 @@ __global__ __launch_bounds__(kWG) void k_decode(const uint8_t *__restrict__ st, 
     const uint64_t tile = tile_of_block(ord, blockIdx.x);
     if (ld_ctl(&ctl[1]) & (kErrStream | kErrTimeout)) {
         // Successors may wait on this tile: publish before leaving.
-        if (!INDEXED && threadIdx.x == 0) st_status(&status[tile], kFlagIncl);
+        if (!INDEXED && threadIdx.x == 0) publish_abort_d(status, g, tile);
         return;
     }
     decode_tile<V, INDEXED, SPB>(st, len, g, blkpos, idx_carry, out, status, ctl, tile, L);
@@ -3356,8 +3466,10 @@ uint64_t decompress_ws_layout(const Geom &g, uint64_t len, uint8_t *base, Decomp
     off += 8 * nt1;
     if (w) w->cnt_st = reinterpret_cast<uint64_t *>(base + off);
     off += 8 * nt1;
+    // Decode look-back: a status per tile, a tile's first word, then the
+    // group statuses.
     if (w) w->dec_st = reinterpret_cast<uint64_t *>(base + off);
-    off += 8 * (g.ntiles ? g.ntiles : 1);
+    off += 8 * (g.ntiles ? 2 * g.ntiles + kGroupStride * lb_groups(g.ntiles) : 1);
     if (w) w->nclr = (off - clr0) / 8;
     off = align_up(off);
     if (w) w->blkpos = reinterpret_cast<uint64_t *>(base + off);
